@@ -1,0 +1,160 @@
+"""Benchmark of the SCRIMP self-join hot path on one MI355X.
+
+Default workload: the plain FP64 drop-in configuration -- one random walk of
+n = 2^20 N(0,1) steps (seed 1), window m = 256, exclusion m/4 -- computed by the
+exact FP64 path with the series already resident in HBM.  One step is one full
+self-join (statistics, seeds, all diagonals, merge, unpack to profile/index).
+
+Prints ONE JSON line.  The metric is matrix-profile cells/s with
+cells = W(W+1)/2, W = n - m + 1 - m/4 - 1 (the reference logs (P-ez)^2/t, about
+twice that).  `roofline` is the dominant kernel's arithmetic rate (8 flops per
+cell: score 2 mul + 1 FMA, recurrence 2 FMA) over the MI355X FP64 vector peak,
+from HIP events on the kernel's own stream; `cpu_baseline` times the oracle's
+restatement of scrimp_triv_par (AVX2 ABBA kernel, one thread per rank) on a
+bounded sample of the same workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+FLOPS_PER_CELL = 8
+
+WORKLOADS = {
+    # name: (generator, n, m, precision)
+    "cfg2-f64": ("walk", 1, 1 << 20, 256, "f64"),
+    "cfg2-f32r": ("walk", 1, 1 << 20, 256, "f32r"),
+    "cfg1-f64": ("walk", 0, 65536, 400, "f64"),
+    "cfg3-f32r": ("precision", 2, 1 << 22, 1024, "f32r"),
+    "cfg3-f64": ("precision", 2, 1 << 22, 1024, "f64"),
+}
+
+
+def make_series(kind, seed, n, m):
+    import thesiscode_amd as tc
+
+    if kind == "walk":
+        return tc.random_walk(seed, n)
+    return tc.precision_series(seed, n, m)[0]
+
+
+def run_gpu(workload: str, steps: int, warmup: int, band_k: int, tile_len: int):
+    import thesiscode_amd as tc
+
+    kind, seed, n, m, prec = WORKLOADS[workload]
+    A = make_series(kind, seed, n, m)
+    P = n - m + 1
+    with tc.Device(0) as dev:
+        d_ts = dev.upload(A)
+        d_prof = dev.alloc(8 * P)
+        d_idx = dev.alloc(8 * P)
+        opts = tc.make_opts(band_k=band_k, tile_len=tile_len)
+        for _ in range(warmup):
+            dev.selfjoin(prec, d_ts, n, m, d_prof, d_idx, opts)
+        dev.sync()
+        kern_ms = []
+        total_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            tm = dev.selfjoin(prec, d_ts, n, m, d_prof, d_idx, opts)
+            kern_ms.append(tm["kernel_ms"])
+            total_ms.append(tm["total_ms"])
+        dev.sync()
+        wall = time.perf_counter() - t0
+        prof = dev.download(d_prof, np.empty(P))
+    return {
+        "n": n, "m": m, "precision": prec, "cells": tm["cells"], "wall_s": wall,
+        "kernel_ms": float(np.mean(kern_ms)), "device_ms": float(np.mean(total_ms)),
+        "band_k": tm["band_k"], "tile_len": tm["tile_len"], "work_items": tm["work_items"],
+        "profile_checksum": float(np.sum(prof)),
+    }
+
+
+def cpu_baseline(workload: str, target_s: float = 12.0):
+    """Oracle restatement of scrimp_triv_par on a bounded sample (every k-th block)."""
+    from oracle import oracle as orc
+
+    kind, seed, n, m, _ = WORKLOADS[workload]
+    A = make_series(kind, seed, n, m)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    probe_stride = 256
+    _, _, cells, secs = orc.triv_par(A, m, nranks=threads, nthreads=threads, kernel=1,
+                                     block_stride=probe_stride)
+    rate = cells / max(secs, 1e-6)
+    total = cells * probe_stride
+    stride = int(max(1, min(probe_stride, np.ceil(total / (rate * target_s)))))
+    _, _, cells, secs = orc.triv_par(A, m, nranks=threads, nthreads=threads, kernel=1, block_stride=stride)
+    return {
+        "value": cells / secs, "unit": "cells/s", "cores": threads, "kind": "port",
+        "sample": (f"{workload}: every {stride}th block of 500 diagonals of each of {threads} ranks "
+                   f"({cells:.3e} cells, {secs:.1f} s; stats+seeds+evaluation+reduction)"),
+    }
+
+
+def read_traffic(workload: str):
+    path = ROOT / "profiles" / f"traffic_{workload}.json"
+    if path.exists():
+        return json.loads(path.read_text()).get("hbm_bytes_per_launch")
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg2-f64", choices=sorted(WORKLOADS))
+    ap.add_argument("--band-k", type=int, default=0)
+    ap.add_argument("--tile-len", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variant", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    r = run_gpu(args.workload, args.steps, args.warmup, args.band_k, args.tile_len)
+    ms_per_step = 1000.0 * r["wall_s"] / args.steps
+    value = r["cells"] / (ms_per_step / 1000.0)
+    peak = FP64_VECTOR_PEAK_TFLOPS if r["precision"] == "f64" else FP32_VECTOR_PEAK_TFLOPS
+    achieved = FLOPS_PER_CELL * r["cells"] / (r["kernel_ms"] / 1000.0) / 1e12
+    kernel_name = "k_walk_f64" if r["precision"] == "f64" else "k_tiles_f32"
+    out = {
+        "metric": "matrix-profile cells/s", "value": value, "unit": "cells/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if r["precision"] == "f64" else "f32 recurrence, f64 stats/seeds",
+        "data": "synthetic (project Philox4x32-10 random walk; inputs resident in HBM)",
+        "config": {"workload": args.workload, "n": r["n"], "m": r["m"], "exclusion": r["m"] // 4,
+                   "cells": r["cells"], "precision": r["precision"], "band_k": r["band_k"],
+                   "tile_len": r["tile_len"], "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "kernel": kernel_name, "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": read_traffic(args.workload),
+                     "kernel_ms": r["kernel_ms"], "flops_per_cell": FLOPS_PER_CELL},
+        "device_ms_per_step": r["device_ms"],
+    }
+    if not args.no_variant and args.workload == "cfg2-f64":
+        v = run_gpu("cfg2-f32r", max(1, args.steps // 2), 1, 0, 0)
+        out["variants"] = {"cfg2-f32r": {"value": v["cells"] / (v["wall_s"] / max(1, args.steps // 2)),
+                                         "kernel_ms": v["kernel_ms"], "band_k": v["band_k"],
+                                         "tile_len": v["tile_len"]}}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.workload)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

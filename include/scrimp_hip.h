@@ -1,0 +1,144 @@
+/*
+ * scrimp_hip.h -- C ABI of the MI355X SCRIMP matrix-profile self-join.
+ *
+ * Plain C types only (pointers, sizes, status codes); no C++ or torch types
+ * cross this boundary and no exception escapes it.  Every function returns
+ * MP_OK (0) or a negative MP_ERR_* status.
+ *
+ * Result convention (same as the reference's MatProfSOA,
+ * scrimppp/inc/ScrimpTrivialPar.hpp:16-20): two arrays of length
+ * P = n - m + 1, profile[i] = squared z-normalised Euclidean distance
+ * d^2 = 2m - 2*score of the best match of window i, index[i] = that match's
+ * start or -1 when no admissible window scored above the initial -1.0
+ * (scrimppp/src/ScrimpTrivialPar.cpp:167-168, 336-340).  The exclusion zone
+ * defaults to m/4: pairs with |i-j| <= m/4 are never compared
+ * (scrimppp/src/ScrimpTrivialPar.cpp:149-150, 202, 221-224).
+ */
+#ifndef SCRIMP_HIP_H
+#define SCRIMP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    MP_OK = 0,
+    MP_ERR_BAD_WINDOW = -1,    /* m < 1 or exclusion < 0                                    */
+    MP_ERR_SHORT_SERIES = -2,  /* n < m: profile length <= 0
+                                  (reference throws, ScrimpTrivialPar.cpp:182-184)          */
+    MP_ERR_NULL = -3,          /* a required pointer is NULL                                */
+    MP_ERR_OOM = -4,           /* device or host allocation failed                          */
+    MP_ERR_DEVICE = -5,        /* a HIP runtime call failed                                 */
+    MP_ERR_TOO_LONG = -6,      /* profile length beyond the supported 2^30
+                                  (reference: MAX_PROFILE_LENGTH, ScrimpTrivialPar.cpp:179)  */
+    MP_ERR_BAD_ARG = -7,       /* an option or size is out of range                         */
+    MP_ERR_NO_DEVICE = -8,     /* no HIP device / the library was built without one         */
+    MP_ERR_ADMISSION = -9,     /* request does not fit the device memory budget             */
+    MP_ERR_NOT_FOUND = -10     /* unknown request handle                                    */
+};
+
+const char* mp_status_string(int status);
+
+/* ---- options and timing -------------------------------------------------- */
+typedef struct mp_opts {
+    int32_t exclusion;   /* exclusion zone; -1 selects the reference default m/4            */
+    int32_t band_k;      /* diagonals per lane (band width = 64*band_k); 0 = default       */
+    int32_t tile_len;    /* FP32 path: columns per tile = re-seed interval B; 0 = default  */
+    int32_t refine;      /* FP32 path: 1 = recompute each winner's distance in FP64        */
+    int32_t index_init;  /* index reported when nothing beat the initial score: -1 (the
+                            parallel drivers) or 0 (scrimp_vert_blocked,
+                            ScrimpVertBlocked.cpp:104)                                     */
+    int32_t reserved[3];
+} mp_opts;
+
+void mp_opts_default(mp_opts* o);
+
+typedef struct mp_timing {
+    double total_ms;     /* device time of the whole self-join (HIP events)                */
+    double stats_ms;     /* window statistics                                              */
+    double seeds_ms;     /* FP64 diagonal seeds (exact path)                               */
+    double kernel_ms;    /* diagonal band / tile kernel                                    */
+    double finalize_ms;  /* refine + unpack to profile/index                               */
+    double cells;        /* cells evaluated: W(W+1)/2, W = P - excl - 1                     */
+    int64_t work_items;  /* bands (exact path) or tiles (FP32 path) processed              */
+    int32_t band_k, tile_len;
+} mp_timing;
+
+/* ---- device context (one per thread or stream; calls on one are serialised) */
+typedef struct mp_ctx mp_ctx;
+
+int mp_device_count(int* count);
+int mp_ctx_create(int device, mp_ctx** out);
+int mp_ctx_destroy(mp_ctx* ctx);
+int mp_ctx_sync(mp_ctx* ctx);
+/* plumbing for callers that keep inputs resident in HBM */
+int mp_dev_alloc(mp_ctx* ctx, uint64_t bytes, void** out);
+int mp_dev_free(mp_ctx* ctx, void* p);
+int mp_memcpy_h2d(mp_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int mp_memcpy_d2h(mp_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- self-join, plain FP64 (bit-exact restatement of the reference) -------
+ * Replaces steps 5-8 of scrimp_triv_par::compute_matrix_profile
+ * (scrimppp/src/ScrimpTrivialPar.cpp:263-340): window statistics
+ * (ScrimpSequOpt.cpp:33-79), seeds (ScrimpSequOpt.cpp:81-96), the diagonal
+ * kernel (kernels.cpp:169-306) over all diagonals ez+1..P-1, the rank merge
+ * (ScrimpTrivialPar.cpp:68-108) and the d^2 transform (336-340).
+ * Host pointers; ts has n elements, profile/index have P elements.          */
+int mp_selfjoin_f64(const double* ts, int64_t n, int32_t m, double* profile,
+                    int64_t* index, const mp_opts* opts, mp_timing* timing);
+/* Same on device pointers (d_ts: n doubles, d_profile/d_index: P each). */
+int mp_selfjoin_f64_dev(mp_ctx* ctx, const double* d_ts, int64_t n, int32_t m,
+                        double* d_profile, int64_t* d_index, const mp_opts* opts,
+                        mp_timing* timing);
+
+/* ---- self-join, FP32 recurrence with FP64 statistics and re-seeding -------
+ * Same result convention.  The covariance recurrence runs in FP32; window
+ * statistics use compensated FP64 sums; every tile of tile_len columns is
+ * seeded with an exact FP64 dot product; with opts->refine the reported
+ * distance of each winner is recomputed in FP64.                          */
+int mp_selfjoin_f32r(const double* ts, int64_t n, int32_t m, double* profile,
+                     int64_t* index, const mp_opts* opts, mp_timing* timing);
+int mp_selfjoin_f32r_dev(mp_ctx* ctx, const double* d_ts, int64_t n, int32_t m,
+                         double* d_profile, int64_t* d_index, const mp_opts* opts,
+                         mp_timing* timing);
+
+/* ---- kernel-level drop-in ---------------------------------------------------
+ * Exact 16-argument semantics of matrix_profile::eval_diag_block_triangle
+ * (scrimppp/inc/kernels.h:7-24, scrimppp/src/kernels.cpp:368-410): for every
+ * column j < trianglen and diagonal offset 0 <= i-j < min(blocklen,
+ * trianglen-j), score = tmpQ[i-j]*(s_vert[i]*s_hor[j]) - mu_vert[i]*mu_hor[j];
+ * tmpQ advances by A_vert[i+w]*A_hor[j+w] - A_vert[i]*A_hor[j]; the row
+ * maximum goes to prof_rowmin[i]/idx_rowmin[i] (index j+basecol), the column
+ * maximum to prof_colmin[j]/idx_colmin[j] (index i+baserow), replacing only on
+ * a strictly greater score.  Row and column views may alias one array as in
+ * the reference's self-join adapters (ScrimpTrivialPar.cpp:110-140).  Host
+ * pointers; runs on the device of a library-owned context.  The final tmpQ
+ * update at i = trianglen-1 reads A_vert[trianglen-1+windowSize] like the
+ * reference (kernels.cpp:218-222); that element must be readable.        */
+int mp_eval_diag_block_triangle_f64(double prof_colmin[], int64_t idx_colmin[],
+                                    double prof_rowmin[], int64_t idx_rowmin[],
+                                    double tmpQ[], int64_t blocklen, int64_t trianglen,
+                                    const double A_hor[], const double A_vert[],
+                                    int64_t windowSize, const double s_hor[],
+                                    const double mu_hor[], const double s_vert[],
+                                    const double mu_vert[], int64_t baserow,
+                                    int64_t basecol);
+
+/* ---- synthetic inputs (CPU only, bit-identical on every host) -------------
+ * Counter-based Philox4x32-10 (Salmon et al., SC'11; key = 64-bit seed,
+ * words {seed lo, seed hi}) + Box-Muller.  None of these has a counterpart in
+ * the reference, which reads its series from files (ScrimpSequ.hpp:48-112). */
+void mp_gen_philox4x32_10(const uint32_t ctr[4], uint64_t key, uint32_t out[4]);
+void mp_gen_normals(uint64_t seed, uint32_t stream, int64_t first, int64_t count, double* out);
+void mp_gen_uniforms(uint64_t seed, uint32_t stream, int64_t first, int64_t count, double* out);
+void mp_gen_random_walk(uint64_t seed, uint32_t stream, int64_t n, double* out);
+int mp_gen_precision_series(uint64_t seed, int64_t n, int32_t m, double* out, double* meta);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCRIMP_HIP_H */

@@ -1,0 +1,102 @@
+/*
+ * scrimp_oracle.h -- CPU restatement of the reference SCRIMP self-join.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * or as the timed CPU baseline.  The product library never links it.
+ *
+ * Arithmetic definition (what "the reference" computes, written out because
+ * the reference's bits depend on `-march=native -ffast-math`,
+ * scrimppp/CMakeLists.txt:96-103; GCC contracts a*b+c into one FMA):
+ *   stats  (ScrimpSequOpt.cpp:33-79)   cs[i] = A[i] + cs[i-1];  sq[i] = fma(A[i],A[i],sq[i-1])
+ *                                      sum[k] = cs[m-1+k] - cs[k-1];  mean = sum/m
+ *                                      var = fma(-mean, mean, sumsq/m); sInv = 1/sqrt(var)
+ *                                      mus = (mean*sInv)*sqrt(m)
+ *   seeds  (ScrimpSequOpt.cpp:81-96)   z = fma(A[k+d], A[k], z), k = 0..m-1 in order
+ *   cell   (kernels.cpp:205-222)       score = fma(QT, sv*sh, -(mv*mh))
+ *                                      QT = fma(A[i+m], A[j+m], QT); QT = fma(-A[i], A[j], QT)
+ *   final  (ScrimpTrivialPar.cpp:336-340)  d2 = 2m - 2*score
+ * Seeds are summed sequentially (the reference build may vectorise that sum);
+ * everything else has one fixed evaluation order.
+ */
+#ifndef SCRIMP_ORACLE_H
+#define SCRIMP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* window statistics: sInv[P], mus[P]  (ScrimpSequOpt.cpp:33-79) */
+void orc_window_stats(const double* A, long n, int m, double* sInv, double* mus);
+
+/* seeds qt[d] for d in [first, last] (ScrimpSequOpt.cpp:81-96); qt indexed by d */
+void orc_seeds(const double* A, long n, int m, long first, long last, double* qt);
+
+/* scalar kernel, kernels.cpp:310-366 (the specification of the AVX2 kernel) */
+void orc_eval_block_scalar(double prof_colmin[], long idx_colmin[], double prof_rowmin[],
+                           long idx_rowmin[], double tmpQ[], long blocklen, long trianglen,
+                           const double A_hor[], const double A_vert[], long w,
+                           const double s_hor[], const double mu_hor[], const double s_vert[],
+                           const double mu_vert[], long baserow, long basecol);
+
+/* AVX2+FMA "ABBA" kernel, kernels.cpp:169-306 (4 diagonals per vector, same math) */
+void orc_eval_block_abba(double prof_colmin[], long idx_colmin[], double prof_rowmin[],
+                         long idx_rowmin[], double tmpQ[], long blocklen, long trianglen,
+                         const double A_hor[], const double A_vert[], long w,
+                         const double s_hor[], const double mu_hor[], const double s_vert[],
+                         const double mu_vert[], long baserow, long basecol);
+
+/* 1-D partitioning, partitioning_1d.cpp:8-48 */
+void orc_get_partition(int part_id, int num_partitions, int ids_to_process, int* first, int* last);
+
+/*
+ * scrimp_triv_par driver (ScrimpTrivialPar.cpp:142-368) with `nranks` ranks
+ * executed by `nthreads` POSIX threads (each rank: own result SoA, 2
+ * partitions r and 2p-1-r, blocks of `blocklen`, seeds per block, kernel per
+ * block, d2 transform), followed by the element-wise min reduction of
+ * ScrimpTrivialPar.cpp:68-108 in rank order.  `kernel`: 0 scalar, 1 AVX2.
+ * `excl` < 0 selects m/4.  `block_stride` > 1 evaluates only every
+ * block_stride-th block of each rank (a bounded timing sample); *cells gets
+ * the number of cells evaluated and *seconds the maximum over ranks of the
+ * stats+seeds+evaluation time plus the reduction.  Returns 0, or -1 when
+ * the profile length is <= 0 (the reference throws there).
+ */
+int orc_triv_par(const double* A, long n, int m, long excl, int blocklen, int nranks,
+                 int nthreads, int kernel, int block_stride, double* profile, long* index,
+                 double* cells, double* seconds);
+
+/*
+ * scrimp_vert_blocked driver (ScrimpVertBlocked.cpp:89-186) without the
+ * unseeded shuffle of block starts (order only changes ties): index starts
+ * at 0 and profile at -1.0.
+ */
+int orc_vert_blocked(const double* A, long n, int m, long excl, int blocklen, int kernel,
+                     double* profile, long* index);
+
+/*
+ * Brute force in long double: for every pair |i-j| > excl,
+ * d2 = 2m(1 - rho_ij) with rho from two-pass centred sums; profile[i] =
+ * min over j, index[i] = the first j attaining it.  O(P^2 m).
+ */
+void orc_brute_force(const double* A, long n, int m, long excl, double* profile, long* index);
+
+/* d2 of one pair in long double (two-pass, population sigma) */
+double orc_pair_dist2(const double* A, long n, int m, long i, long j);
+
+/* Window mean and population sigma of every window in long double (two-pass). */
+void orc_window_moments_ld(const double* A, long n, int m, double* mean, double* sigma);
+
+/*
+ * Planted-mistake switch for the oracle's own pinning tests: 0 = none,
+ * 1 = drop the A[i]*A[j] term of the recurrence, 2 = flip the sign of the
+ * mean product in the score, 3 = report the row index for column results,
+ * 4 = exclusion zone one diagonal too narrow, 5 = sample instead of
+ * population variance.
+ */
+void orc_set_fault(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCRIMP_ORACLE_H */

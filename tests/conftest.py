@@ -1,0 +1,67 @@
+"""Shared fixtures and helpers for the SCRIMP tests.
+
+`-m "not gpu"` runs on any host: the oracle's pinning tests, the C ABI's
+exports and the generators.  `-m gpu` are the parity tests proper; they call
+libscrimp_hip.so through its C ABI and compare with the CPU oracle.
+"""
+from __future__ import annotations
+
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X); the parity tests proper")
+
+
+@pytest.fixture(scope="session")
+def golden():
+    return np.load(GOLDEN / "golden_small.npz")
+
+
+def read_ts_bin(path: Path) -> np.ndarray:
+    """Reference time-series file format v2 (scrimppp/inc/biniobase.h:10-21): 100-byte header."""
+    raw = path.read_bytes()
+    fmt, off = np.frombuffer(raw[:8], dtype="<u4").tolist()
+    n = int(np.frombuffer(raw[8:16], dtype="<u8")[0])
+    assert fmt == 2 and off == 100
+    return np.frombuffer(raw[off:off + 8 * n], dtype="<f8").copy()
+
+
+@pytest.fixture(scope="session")
+def test_series() -> np.ndarray:
+    return read_ts_bin(GOLDEN / "test_series.bin")
+
+
+def cpu_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def golden_cases(golden):
+    names = sorted({k.split("__")[0] for k in golden.files if k.endswith("__series")})
+    return [(n, golden[f"{n}__series"], int(golden[f"{n}__m"])) for n in names]
+
+
+def assert_indices_match(A, m, prof_ref, idx_ref, idx_got, *, abs_tol, min_frac=0.999):
+    """Indices must agree except where both partners are a near tie.
+
+    A mismatch is accepted when the true (long double) d^2 of the two partners
+    differ by at most abs_tol; at most 1-min_frac of the entries may mismatch.
+    """
+    from oracle import oracle as orc
+
+    bad = np.nonzero(idx_ref != idx_got)[0]
+    assert bad.size <= max(1, int((1.0 - min_frac) * idx_ref.size)), f"{bad.size} index mismatches"
+    for i in bad[:200]:
+        assert idx_got[i] >= 0, (i, idx_ref[i], idx_got[i])
+        d_ref = orc.pair_dist2(A, m, i, idx_ref[i])
+        d_got = orc.pair_dist2(A, m, i, idx_got[i])
+        assert abs(d_ref - d_got) <= abs_tol, (i, idx_ref[i], idx_got[i], d_ref, d_got, prof_ref[i])

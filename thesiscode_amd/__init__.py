@@ -1,0 +1,183 @@
+"""MI355X-native SCRIMP matrix-profile self-join (Python view of the C ABI).
+
+Results follow the reference's MatProfSOA (scrimppp/inc/ScrimpTrivialPar.hpp:16-20):
+``profile[i]`` is the squared z-normalised distance ``2m - 2*score`` of window
+i's best match and ``index[i]`` its start (-1 when nothing beat the initial
+score).  The exclusion zone defaults to ``m // 4`` (ScrimpTrivialPar.cpp:150).
+
+Every call runs on the HIP device through libscrimp_hip.so; there is no CPU
+fallback, and a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._ffi import Opts, ScrimpError, Timing, check, lib
+
+__all__ = [
+    "ScrimpError", "Device", "selfjoin_f64", "selfjoin_f32r", "eval_diag_block_triangle_f64",
+    "random_walk", "normals", "uniforms", "precision_series", "profile_length", "exclusion_zone",
+    "cells", "ALGORITHMS", "make_opts",
+]
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _lptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def exclusion_zone(m: int) -> int:
+    return m // 4
+
+
+def profile_length(n: int, m: int) -> int:
+    return n - m + 1
+
+
+def cells(n: int, m: int, excl: int | None = None) -> int:
+    """Cells one self-join evaluates: W(W+1)/2 with W = P - excl - 1."""
+    excl = m // 4 if excl is None else excl
+    w = max(0, profile_length(n, m) - excl - 1)
+    return w * (w + 1) // 2
+
+
+def make_opts(exclusion: int = -1, band_k: int = 0, tile_len: int = 0, refine: bool = True,
+              index_init: int = -1) -> Opts:
+    o = Opts()
+    lib().mp_opts_default(C.byref(o))
+    o.exclusion = exclusion
+    o.band_k = band_k
+    o.tile_len = tile_len
+    o.refine = 1 if refine else 0
+    o.index_init = index_init
+    return o
+
+
+def _selfjoin(fn_name: str, ts, m: int, opts: Opts):
+    ts = np.ascontiguousarray(ts, dtype=np.float64)
+    n = ts.shape[0]
+    P = max(0, n - m + 1)
+    prof = np.empty(P, dtype=np.float64)
+    idx = np.empty(P, dtype=np.int64)
+    tm = Timing()
+    rc = getattr(lib(), fn_name)(_dptr(ts), n, m, _dptr(prof), _lptr(idx), C.byref(opts), C.byref(tm))
+    check(rc, fn_name)
+    return prof, idx, tm.as_dict()
+
+
+def selfjoin_f64(ts, m: int, *, exclusion: int = -1, band_k: int = 0, index_init: int = -1):
+    """Plain FP64 self-join, bit-identical to the reference arithmetic."""
+    return _selfjoin("mp_selfjoin_f64", ts, m, make_opts(exclusion, band_k, 0, True, index_init))
+
+
+def selfjoin_f32r(ts, m: int, *, exclusion: int = -1, band_k: int = 0, tile_len: int = 0,
+                  refine: bool = True, index_init: int = -1):
+    """FP32 recurrence, FP64 statistics and per-tile FP64 seeds (tile_len = re-seed interval)."""
+    return _selfjoin("mp_selfjoin_f32r", ts, m, make_opts(exclusion, band_k, tile_len, refine, index_init))
+
+
+def eval_diag_block_triangle_f64(prof_colmin, idx_colmin, prof_rowmin, idx_rowmin, tmpQ, blocklen,
+                                 trianglen, A_hor, A_vert, window, s_hor, mu_hor, s_vert, mu_vert,
+                                 baserow, basecol):
+    """In-place drop-in of matrix_profile::eval_diag_block_triangle (kernels.h:7-24).
+
+    Array arguments are float64/int64 numpy arrays or views of them; row and
+    column views may alias one array exactly as the reference's adapters do.
+    """
+    for a, dt in ((prof_colmin, np.float64), (prof_rowmin, np.float64), (tmpQ, np.float64),
+                  (idx_colmin, np.int64), (idx_rowmin, np.int64)):
+        if a.dtype != dt or not a.flags.c_contiguous:
+            raise TypeError("output arrays must be contiguous float64/int64")
+    ins = [np.ascontiguousarray(x, dtype=np.float64) for x in (A_hor, A_vert, s_hor, mu_hor, s_vert, mu_vert)]
+    rc = lib().mp_eval_diag_block_triangle_f64(
+        _dptr(prof_colmin), _lptr(idx_colmin), _dptr(prof_rowmin), _lptr(idx_rowmin), _dptr(tmpQ),
+        int(blocklen), int(trianglen), _dptr(ins[0]), _dptr(ins[1]), int(window), _dptr(ins[2]),
+        _dptr(ins[3]), _dptr(ins[4]), _dptr(ins[5]), int(baserow), int(basecol))
+    check(rc, "mp_eval_diag_block_triangle_f64")
+
+
+ALGORITHMS = {"scrimp_hip_f64": selfjoin_f64, "scrimp_hip_f32r": selfjoin_f32r}
+
+
+class Device:
+    """A library context on one GPU, for inputs kept resident in HBM."""
+
+    def __init__(self, device: int = 0):
+        self._ctx = C.c_void_p()
+        check(lib().mp_ctx_create(device, C.byref(self._ctx)), "mp_ctx_create")
+        self._bufs: list[int] = []
+
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        check(lib().mp_dev_alloc(self._ctx, nbytes, C.byref(p)), "mp_dev_alloc")
+        self._bufs.append(p.value)
+        return p.value
+
+    def upload(self, arr: np.ndarray) -> int:
+        arr = np.ascontiguousarray(arr)
+        p = self.alloc(arr.nbytes)
+        check(lib().mp_memcpy_h2d(self._ctx, C.c_void_p(p), arr.ctypes.data_as(C.c_void_p), arr.nbytes), "h2d")
+        return p
+
+    def download(self, ptr: int, out: np.ndarray) -> np.ndarray:
+        check(lib().mp_memcpy_d2h(self._ctx, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), out.nbytes), "d2h")
+        return out
+
+    def selfjoin(self, precision: str, d_ts: int, n: int, m: int, d_prof: int, d_idx: int,
+                 opts: Opts | None = None) -> dict:
+        fn = {"f64": "mp_selfjoin_f64_dev", "f32r": "mp_selfjoin_f32r_dev"}[precision]
+        opts = opts or make_opts()
+        tm = Timing()
+        rc = getattr(lib(), fn)(self._ctx, C.c_void_p(d_ts), n, m, C.c_void_p(d_prof), C.c_void_p(d_idx),
+                                C.byref(opts), C.byref(tm))
+        check(rc, fn)
+        return tm.as_dict()
+
+    def sync(self) -> None:
+        check(lib().mp_ctx_sync(self._ctx), "mp_ctx_sync")
+
+    def close(self) -> None:
+        if self._ctx:
+            for p in self._bufs:
+                lib().mp_dev_free(self._ctx, C.c_void_p(p))
+            self._bufs = []
+            lib().mp_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+# ---- synthetic inputs (CPU, bit-identical on every host) --------------------
+def random_walk(seed: int, n: int, stream: int = 0) -> np.ndarray:
+    out = np.empty(n, dtype=np.float64)
+    lib().mp_gen_random_walk(seed, stream, n, _dptr(out))
+    return out
+
+
+def normals(seed: int, count: int, stream: int = 0, first: int = 0) -> np.ndarray:
+    out = np.empty(count, dtype=np.float64)
+    lib().mp_gen_normals(seed, stream, first, count, _dptr(out))
+    return out
+
+
+def uniforms(seed: int, count: int, stream: int = 0, first: int = 0) -> np.ndarray:
+    out = np.empty(count, dtype=np.float64)
+    lib().mp_gen_uniforms(seed, stream, first, count, _dptr(out))
+    return out
+
+
+def precision_series(seed: int, n: int, m: int):
+    """Random walk + 3 sinusoids + 20 Gaussian bumps; returns (series, meta)."""
+    out = np.empty(n, dtype=np.float64)
+    meta = np.empty(26, dtype=np.float64)
+    check(lib().mp_gen_precision_series(seed, n, m, _dptr(out), _dptr(meta)), "mp_gen_precision_series")
+    return out, {"periods": meta[0:3].copy(), "phases": meta[3:6].copy(), "bump_starts": meta[6:26].astype(np.int64)}

@@ -1,0 +1,672 @@
+// C ABI of the SCRIMP HIP library: contexts, the exact FP64 self-join, the
+// FP32-reseeded self-join and the kernel-level drop-in of
+// matrix_profile::eval_diag_block_triangle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "reseeded_f32.hpp"
+#include "scrimp_hip.h"
+
+namespace mp {
+// launchers defined in exact_f64.hip
+int launch_ref_stats(const double* A, int n, int m, double* cs, double* sq, double* sInv,
+                     double* mus, hipStream_t st);
+int launch_ref_seeds(const double* A, int base, int count, int m, double* qt, hipStream_t st);
+int launch_fill_u64(unsigned long long* p, int64_t count, unsigned long long v, hipStream_t st);
+int launch_fill_u32(unsigned int* p, int count, unsigned int v, hipStream_t st);
+int launch_unpack_f64(const unsigned long long* val, const unsigned long long* key, int P, int m,
+                      int ib, int64_t index_init, double* profile, int64_t* index,
+                      hipStream_t st);
+int launch_pack_block(const double* prof, int count, int ib, unsigned long long* val,
+                      unsigned long long* key, hipStream_t st);
+int launch_unpack_block(const unsigned long long* val, const unsigned long long* key, int count,
+                        int ib, int64_t pmin, const int64_t* idx_in, double* prof_out,
+                        int64_t* idx_out, hipStream_t st);
+struct WalkF64Launch {
+    const double *Ah, *sh, *mh, *Av, *sv, *mv;
+    double* qt;
+    unsigned long long *rowval, *rowkey, *colval, *colkey;
+    unsigned int* counter;
+    int64_t row_off, col_off;
+    int blocklen, tl, m, ib, writeback, band_k, groups;
+};
+int walk_f64_nbands(int blocklen, int band_k);
+int launch_walk_f64(const WalkF64Launch& a, hipStream_t st);
+}  // namespace mp
+
+#define MP_TRY(expr)                       \
+    do {                                   \
+        int mp_rc_ = (expr);               \
+        if (mp_rc_ != MP_OK) return mp_rc_; \
+    } while (0)
+#define MP_HIP(expr)                                   \
+    do {                                               \
+        if ((expr) != hipSuccess) return MP_ERR_DEVICE; \
+    } while (0)
+
+namespace {
+
+constexpr int64_t kMaxProfile = int64_t(1) << 30;
+
+// Grow-only device arena owned by a context; sub-allocations are 256-byte aligned.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    int reserve(size_t bytes) {
+        if (bytes <= cap) return MP_OK;
+        if (base) (void)hipFree(base);
+        base = nullptr;
+        cap = 0;
+        if (hipMalloc(&base, bytes) != hipSuccess) {
+            base = nullptr;
+            return MP_ERR_OOM;
+        }
+        cap = bytes;
+        return MP_OK;
+    }
+    template <class T>
+    T* take(size_t count) {
+        const size_t b = (count * sizeof(T) + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + used);
+        used += b;
+        return p;
+    }
+};
+
+size_t aligned_bytes(size_t count, size_t elem) { return (count * elem + 255) & ~size_t(255); }
+
+}  // namespace
+
+struct mp_ctx {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    Arena work;   // per-call scratch
+    Arena io;     // staging for host-pointer calls
+    std::mutex lock;
+};
+
+namespace {
+
+int ctx_init(mp_ctx* c, int device) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return MP_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return MP_ERR_BAD_ARG;
+    c->device = device;
+    MP_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    MP_HIP(hipGetDeviceProperties(&prop, device));
+    c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    MP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) MP_HIP(hipEventCreate(&e));
+    return MP_OK;
+}
+
+std::mutex g_default_lock;
+mp_ctx* g_default = nullptr;
+
+int default_ctx(mp_ctx** out) {
+    std::lock_guard<std::mutex> g(g_default_lock);
+    if (!g_default) {
+        mp_ctx* c = new (std::nothrow) mp_ctx();
+        if (!c) return MP_ERR_OOM;
+        const int rc = ctx_init(c, 0);
+        if (rc != MP_OK) {
+            delete c;
+            return rc;
+        }
+        g_default = c;
+    }
+    *out = g_default;
+    return MP_OK;
+}
+
+int bits_for(int64_t values) {
+    int b = 1;
+    while ((int64_t(1) << b) < values) ++b;
+    return b;
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return static_cast<double>(ms);
+}
+
+struct Geometry {
+    int64_t n, P;
+    int m, excl, base, W;
+    double cells;
+};
+
+int geometry(int64_t n, int32_t m, const mp_opts* o, Geometry* g) {
+    if (m < 1) return MP_ERR_BAD_WINDOW;
+    const int excl = (o && o->exclusion >= 0) ? o->exclusion : m / 4;
+    if (o && o->exclusion < -1) return MP_ERR_BAD_WINDOW;
+    if (n < m) return MP_ERR_SHORT_SERIES;
+    const int64_t P = n - m + 1;
+    if (P > kMaxProfile || n > kMaxProfile + 4096) return MP_ERR_TOO_LONG;
+    g->n = n;
+    g->P = P;
+    g->m = m;
+    g->excl = excl;
+    g->base = excl + 1;
+    const int64_t W = P - g->base;
+    g->W = W > 0 ? static_cast<int>(W) : 0;
+    g->cells = 0.5 * static_cast<double>(g->W) * static_cast<double>(g->W + 1);
+    return MP_OK;
+}
+
+int pick_band_k_f64(const mp_ctx* c, int W, int requested) {
+    if (requested > 0) return requested;
+    int k = 4;
+    const int target = 2 * 4 * c->cus;  // two waves per SIMD
+    while (k > 1 && (mp::walk_f64_nbands(W, k) + 1) / 2 < target) k >>= 1;
+    return k;
+}
+
+// ord(-1.0) in the order-preserving uint64 map used by the exact path's keys
+unsigned long long init_val_f64() {
+    const double x = -1.0;
+    uint64_t b;
+    std::memcpy(&b, &x, sizeof b);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mp_status_string(int s) {
+    switch (s) {
+        case MP_OK: return "ok";
+        case MP_ERR_BAD_WINDOW: return "bad window length or exclusion zone";
+        case MP_ERR_SHORT_SERIES: return "time series shorter than the window: profile length <= 0";
+        case MP_ERR_NULL: return "null pointer";
+        case MP_ERR_OOM: return "out of memory";
+        case MP_ERR_DEVICE: return "HIP runtime error";
+        case MP_ERR_TOO_LONG: return "profile length exceeds the supported maximum";
+        case MP_ERR_BAD_ARG: return "argument out of range";
+        case MP_ERR_NO_DEVICE: return "no HIP device available";
+        case MP_ERR_ADMISSION: return "request exceeds the device memory budget";
+        case MP_ERR_NOT_FOUND: return "unknown request";
+        default: return "unknown status";
+    }
+}
+
+void mp_opts_default(mp_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->exclusion = -1;
+    o->refine = 1;
+    o->index_init = -1;
+}
+
+int mp_device_count(int* count) {
+    if (!count) return MP_ERR_NULL;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return MP_OK;
+}
+
+int mp_ctx_create(int device, mp_ctx** out) {
+    if (!out) return MP_ERR_NULL;
+    mp_ctx* c = new (std::nothrow) mp_ctx();
+    if (!c) return MP_ERR_OOM;
+    const int rc = ctx_init(c, device);
+    if (rc != MP_OK) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return MP_OK;
+}
+
+int mp_ctx_destroy(mp_ctx* c) {
+    if (!c) return MP_ERR_NULL;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->work.base) (void)hipFree(c->work.base);
+    if (c->io.base) (void)hipFree(c->io.base);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MP_OK;
+}
+
+int mp_ctx_sync(mp_ctx* c) {
+    if (!c) return MP_ERR_NULL;
+    MP_HIP(hipSetDevice(c->device));
+    MP_HIP(hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+int mp_dev_alloc(mp_ctx* c, uint64_t bytes, void** out) {
+    if (!c || !out) return MP_ERR_NULL;
+    MP_HIP(hipSetDevice(c->device));
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return MP_ERR_OOM;
+    return MP_OK;
+}
+
+int mp_dev_free(mp_ctx* c, void* p) {
+    if (!c) return MP_ERR_NULL;
+    MP_HIP(hipSetDevice(c->device));
+    MP_HIP(hipFree(p));
+    return MP_OK;
+}
+
+int mp_memcpy_h2d(mp_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || !dst || !src) return MP_ERR_NULL;
+    MP_HIP(hipSetDevice(c->device));
+    MP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    MP_HIP(hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+int mp_memcpy_d2h(mp_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || !dst || !src) return MP_ERR_NULL;
+    MP_HIP(hipSetDevice(c->device));
+    MP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    MP_HIP(hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// exact FP64 self-join
+// ---------------------------------------------------------------------------
+static int selfjoin_f64_locked(mp_ctx* c, const double* d_ts, const Geometry& g, double* d_profile,
+                               int64_t* d_index, const mp_opts* o, mp_timing* tm) {
+    hipStream_t st = c->stream;
+    const int64_t n = g.n, P = g.P;
+    const int m = g.m;
+    const int64_t index_init = (o && o->index_init == 0) ? 0 : -1;
+    const int W = g.W;
+    const int band_k = pick_band_k_f64(c, W, o ? o->band_k : 0);
+    if (band_k != 1 && band_k != 2 && band_k != 4 && band_k != 8) return MP_ERR_BAD_ARG;
+    const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n, 8) +
+                        2 * aligned_bytes(P + 64, 8) + aligned_bytes(W + 1, 8) +
+                        2 * aligned_bytes(P, 8) + aligned_bytes(1, 4);
+    MP_TRY(c->work.reserve(need));
+    c->work.used = 0;
+    double* A = c->work.take<double>(n + 64);
+    double* cs = c->work.take<double>(n);
+    double* sq = c->work.take<double>(n);
+    double* sInv = c->work.take<double>(P + 64);
+    double* mus = c->work.take<double>(P + 64);
+    double* qt = c->work.take<double>(W + 1);
+    auto* val = c->work.take<unsigned long long>(P);
+    auto* key = c->work.take<unsigned long long>(P);
+    auto* counter = c->work.take<unsigned int>(1);
+    const int ib = bits_for(P + 1);
+
+    MP_HIP(hipEventRecord(c->ev[0], st));
+    MP_HIP(hipMemcpyAsync(A, d_ts, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    MP_TRY(mp::launch_fill_u64(reinterpret_cast<unsigned long long*>(A + n), 64, 0ull, st));
+    MP_TRY(mp::launch_ref_stats(A, static_cast<int>(n), m, cs, sq, sInv, mus, st));
+    MP_HIP(hipEventRecord(c->ev[1], st));
+    MP_TRY(mp::launch_ref_seeds(A, g.base, W, m, qt, st));
+    const unsigned long long v0 = init_val_f64();
+    MP_TRY(mp::launch_fill_u64(val, P, v0, st));
+    MP_TRY(mp::launch_fill_u64(key, P, v0 | ((1ull << ib) - 1ull), st));
+    MP_TRY(mp::launch_fill_u32(counter, 1, 0u, st));
+    MP_HIP(hipEventRecord(c->ev[2], st));
+    if (W > 0) {
+        mp::WalkF64Launch a{};
+        a.Ah = A;
+        a.sh = sInv;
+        a.mh = mus;
+        a.Av = A + g.base;
+        a.sv = sInv + g.base;
+        a.mv = mus + g.base;
+        a.qt = qt;
+        a.rowval = val + g.base;
+        a.rowkey = key + g.base;
+        a.colval = val;
+        a.colkey = key;
+        a.counter = counter;
+        a.row_off = 0;
+        a.col_off = g.base;
+        a.blocklen = W;
+        a.tl = W;
+        a.m = m;
+        a.ib = ib;
+        a.writeback = 0;
+        a.band_k = band_k;
+        const int nb = mp::walk_f64_nbands(W, band_k);
+        const int items = (nb + 1) / 2;
+        a.groups = std::max(1, std::min((items + 3) / 4, 8 * c->cus));
+        MP_TRY(mp::launch_walk_f64(a, st));
+    }
+    MP_HIP(hipEventRecord(c->ev[3], st));
+    MP_TRY(mp::launch_unpack_f64(val, key, static_cast<int>(P), m, ib, index_init, d_profile,
+                                 d_index, st));
+    MP_HIP(hipEventRecord(c->ev[4], st));
+    MP_HIP(hipEventSynchronize(c->ev[4]));
+    if (tm) {
+        std::memset(tm, 0, sizeof(*tm));
+        tm->stats_ms = elapsed(c->ev[0], c->ev[1]);
+        tm->seeds_ms = elapsed(c->ev[1], c->ev[2]);
+        tm->kernel_ms = elapsed(c->ev[2], c->ev[3]);
+        tm->finalize_ms = elapsed(c->ev[3], c->ev[4]);
+        tm->total_ms = elapsed(c->ev[0], c->ev[4]);
+        tm->cells = g.cells;
+        tm->work_items = W > 0 ? mp::walk_f64_nbands(W, band_k) : 0;
+        tm->band_k = band_k;
+        tm->tile_len = 0;
+    }
+    return MP_OK;
+}
+
+int mp_selfjoin_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double* d_profile,
+                        int64_t* d_index, const mp_opts* o, mp_timing* tm) {
+    if (!c || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
+    Geometry g;
+    MP_TRY(geometry(n, m, o, &g));
+    std::lock_guard<std::mutex> guard(c->lock);
+    MP_HIP(hipSetDevice(c->device));
+    return selfjoin_f64_locked(c, d_ts, g, d_profile, d_index, o, tm);
+}
+
+// Host-pointer wrapper shared by both self-joins.
+using DevSelfJoin = int (*)(mp_ctx*, const double*, int64_t, int32_t, double*, int64_t*,
+                            const mp_opts*, mp_timing*);
+
+static int host_selfjoin(DevSelfJoin fn, const double* ts, int64_t n, int32_t m, double* profile,
+                         int64_t* index, const mp_opts* o, mp_timing* tm) {
+    if (!ts || !profile || !index) return MP_ERR_NULL;
+    Geometry g;
+    MP_TRY(geometry(n, m, o, &g));
+    mp_ctx* c = nullptr;
+    MP_TRY(default_ctx(&c));
+    double* dts;
+    double* dprof;
+    int64_t* didx;
+    {
+        std::lock_guard<std::mutex> guard(c->lock);
+        MP_HIP(hipSetDevice(c->device));
+        MP_TRY(c->io.reserve(aligned_bytes(n, 8) + 2 * aligned_bytes(g.P, 8)));
+        c->io.used = 0;
+        dts = c->io.take<double>(n);
+        dprof = c->io.take<double>(g.P);
+        didx = c->io.take<int64_t>(g.P);
+        MP_HIP(hipMemcpyAsync(dts, ts, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    MP_TRY(fn(c, dts, n, m, dprof, didx, o, tm));
+    std::lock_guard<std::mutex> guard(c->lock);
+    MP_HIP(hipMemcpyAsync(profile, dprof, sizeof(double) * g.P, hipMemcpyDeviceToHost, c->stream));
+    MP_HIP(hipMemcpyAsync(index, didx, sizeof(int64_t) * g.P, hipMemcpyDeviceToHost, c->stream));
+    MP_HIP(hipStreamSynchronize(c->stream));
+    return MP_OK;
+}
+
+int mp_selfjoin_f64(const double* ts, int64_t n, int32_t m, double* profile, int64_t* index,
+                    const mp_opts* o, mp_timing* tm) {
+    return host_selfjoin(mp_selfjoin_f64_dev, ts, n, m, profile, index, o, tm);
+}
+
+// ---------------------------------------------------------------------------
+// FP32-reseeded self-join (a batch of one series)
+// ---------------------------------------------------------------------------
+static int default_tile_len(int m, int requested) {
+    if (requested > 0) return requested;
+    int t = 4 * m;
+    if (t < 1024) t = 1024;
+    if (t > 8192) t = 8192;
+    return (t + 15) & ~15;
+}
+
+int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double* d_profile,
+                         int64_t* d_index, const mp_opts* o, mp_timing* tm) {
+    if (!c || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
+    Geometry g;
+    MP_TRY(geometry(n, m, o, &g));
+    const int band_k = (o && o->band_k > 0) ? o->band_k : 8;
+    if (band_k != 4 && band_k != 8 && band_k != 16) return MP_ERR_BAD_ARG;
+    const int T = default_tile_len(m, o ? o->tile_len : 0);
+    if (T % 16 != 0 || T <= 0) return MP_ERR_BAD_ARG;
+    const int refine = o ? (o->refine != 0) : 1;
+    const int64_t index_init = (o && o->index_init == 0) ? 0 : -1;
+    const int BW = 64 * band_k;
+    const int nbands = (g.W + BW - 1) / BW;
+    std::vector<int64_t> bprefix(static_cast<size_t>(nbands) + 1, 0);
+    for (int b = 0; b < nbands; ++b) {
+        const int64_t L = static_cast<int64_t>(g.W) - static_cast<int64_t>(b) * BW;
+        bprefix[b + 1] = bprefix[b] + (L + T - 1) / T;
+    }
+    const int64_t ntiles = bprefix[nbands];
+    mp::SeriesDesc desc{};
+    desc.off = 0;
+    desc.n = static_cast<int>(n);
+    desc.m = m;
+    desc.P = static_cast<int>(g.P);
+    desc.base = g.base;
+    desc.tl = g.W;
+    desc.nbands = nbands;
+    desc.band_off = 0;
+    desc.init_c1 = static_cast<float>(1.0 - 1.0 / static_cast<double>(m));
+    const int64_t eprefix[2] = {0, g.P};
+    const int64_t tprefix[2] = {0, ntiles};
+
+    std::lock_guard<std::mutex> guard(c->lock);
+    MP_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n + 64, 8) +
+                        3 * aligned_bytes(n + 64, 4) + aligned_bytes(n + 64, 8) +
+                        aligned_bytes(1, sizeof(mp::SeriesDesc)) + 2 * aligned_bytes(2, 8) +
+                        aligned_bytes(nbands + 1, 8) + aligned_bytes(1, 4);
+    MP_TRY(c->work.reserve(need));
+    c->work.used = 0;
+    double* A = c->work.take<double>(n + 64);
+    double* mu = c->work.take<double>(n + 64);
+    double* sigma = c->work.take<double>(n + 64);
+    float* df = c->work.take<float>(n + 64);
+    float* dg = c->work.take<float>(n + 64);
+    float* inv = c->work.take<float>(n + 64);
+    auto* keys = c->work.take<unsigned long long>(n + 64);
+    auto* ddesc = c->work.take<mp::SeriesDesc>(1);
+    auto* dep = c->work.take<int64_t>(2);
+    auto* dtp = c->work.take<int64_t>(2);
+    auto* dbp = c->work.take<int64_t>(nbands + 1);
+    auto* counter = c->work.take<unsigned int>(1);
+
+    MP_HIP(hipEventRecord(c->ev[0], st));
+    MP_HIP(hipMemcpyAsync(ddesc, &desc, sizeof(desc), hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(dep, eprefix, sizeof(eprefix), hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(dtp, tprefix, sizeof(tprefix), hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(dbp, bprefix.data(), sizeof(int64_t) * bprefix.size(),
+                          hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(A, d_ts, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    MP_TRY(mp::launch_fill_u64(reinterpret_cast<unsigned long long*>(A + n), 64, 0ull, st));
+    mp::F32rBatch b{};
+    b.descs = ddesc;
+    b.entry_prefix = dep;
+    b.tile_prefix = dtp;
+    b.band_prefix = dbp;
+    b.nseries = 1;
+    b.nentries = g.P;
+    b.ntiles = ntiles;
+    b.band_k = band_k;
+    b.tile_len = T;
+    b.A = A;
+    b.mu = mu;
+    b.sigma = sigma;
+    b.df = df;
+    b.dg = dg;
+    b.inv = inv;
+    b.keys = keys;
+    b.counter = counter;
+    MP_TRY(mp::launch_f32r_prep(b, st));
+    MP_TRY(mp::launch_fill_u32(counter, 1, 0u, st));
+    MP_HIP(hipEventRecord(c->ev[1], st));
+    MP_HIP(hipEventRecord(c->ev[2], st));
+    if (ntiles > 0) {
+        const int groups = static_cast<int>(std::min<int64_t>((ntiles + 3) / 4, 8 * c->cus));
+        MP_TRY(mp::launch_f32r_tiles(b, groups, st));
+    }
+    MP_HIP(hipEventRecord(c->ev[3], st));
+    MP_TRY(mp::launch_f32r_unpack(b, refine, index_init, d_profile, d_index, st));
+    MP_HIP(hipEventRecord(c->ev[4], st));
+    MP_HIP(hipEventSynchronize(c->ev[4]));
+    if (tm) {
+        std::memset(tm, 0, sizeof(*tm));
+        tm->stats_ms = elapsed(c->ev[0], c->ev[1]);
+        tm->seeds_ms = 0.0;
+        tm->kernel_ms = elapsed(c->ev[2], c->ev[3]);
+        tm->finalize_ms = elapsed(c->ev[3], c->ev[4]);
+        tm->total_ms = elapsed(c->ev[0], c->ev[4]);
+        tm->cells = g.cells;
+        tm->work_items = ntiles;
+        tm->band_k = band_k;
+        tm->tile_len = T;
+    }
+    return MP_OK;
+}
+
+int mp_selfjoin_f32r(const double* ts, int64_t n, int32_t m, double* profile, int64_t* index,
+                     const mp_opts* o, mp_timing* tm) {
+    return host_selfjoin(mp_selfjoin_f32r_dev, ts, n, m, profile, index, o, tm);
+}
+
+// ---------------------------------------------------------------------------
+// kernel-level drop-in of eval_diag_block_triangle
+// ---------------------------------------------------------------------------
+int mp_eval_diag_block_triangle_f64(double prof_colmin[], int64_t idx_colmin[],
+                                    double prof_rowmin[], int64_t idx_rowmin[], double tmpQ[],
+                                    int64_t blocklen, int64_t trianglen, const double A_hor[],
+                                    const double A_vert[], int64_t w, const double s_hor[],
+                                    const double mu_hor[], const double s_vert[],
+                                    const double mu_vert[], int64_t baserow, int64_t basecol) {
+    if (!prof_colmin || !idx_colmin || !prof_rowmin || !idx_rowmin || !tmpQ || !A_hor ||
+        !A_vert || !s_hor || !mu_hor || !s_vert || !mu_vert)
+        return MP_ERR_NULL;
+    if (w < 1) return MP_ERR_BAD_WINDOW;
+    if (trianglen <= 0 || blocklen <= 0) return MP_OK;
+    if (trianglen > kMaxProfile || baserow < 0 || basecol < 0) return MP_ERR_BAD_ARG;
+    const int tl = static_cast<int>(trianglen);
+    const int bl = static_cast<int>(std::min<int64_t>(blocklen, trianglen));
+    // Row and column views either alias one array (same element offset in the
+    // profile and index arrays) or are disjoint.
+    const int64_t pdiff = prof_rowmin - prof_colmin;
+    const int64_t idiff = idx_rowmin - idx_colmin;
+    const bool overlap = (prof_rowmin < prof_colmin + tl) && (prof_colmin < prof_rowmin + tl);
+    const bool ioverlap = (idx_rowmin < idx_colmin + tl) && (idx_colmin < idx_rowmin + tl);
+    if (overlap != ioverlap || (overlap && pdiff != idiff)) return MP_ERR_BAD_ARG;
+    int64_t lo_off = 0, span = tl;  // union range relative to prof_colmin
+    if (overlap) {
+        lo_off = std::min<int64_t>(0, pdiff);
+        span = std::max<int64_t>(tl, pdiff + tl) - lo_off;
+    }
+    const int64_t pmin = std::min(baserow, basecol);
+    const int64_t prange = std::max(baserow, basecol) + tl - pmin;
+    const int ib = bits_for(prange + 1);
+    if (ib > 40) return MP_ERR_BAD_ARG;
+
+    mp_ctx* c = nullptr;
+    MP_TRY(default_ctx(&c));
+    std::lock_guard<std::mutex> guard(c->lock);
+    MP_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int64_t na = tl + w;  // A arrays: indices 0 .. tl-1+w
+    const int64_t nseg = overlap ? 1 : 2;
+    const size_t need = 2 * aligned_bytes(na + 1, 8) + 4 * aligned_bytes(tl, 8) +
+                        aligned_bytes(bl, 8) +
+                        nseg * (2 * aligned_bytes(span, 8) + 3 * aligned_bytes(span, 8)) +
+                        aligned_bytes(1, 4);
+    MP_TRY(c->work.reserve(need));
+    c->work.used = 0;
+    double* dAh = c->work.take<double>(na + 1);
+    double* dAv = c->work.take<double>(na + 1);
+    double* dsh = c->work.take<double>(tl);
+    double* dmh = c->work.take<double>(tl);
+    double* dsv = c->work.take<double>(tl);
+    double* dmv = c->work.take<double>(tl);
+    double* dq = c->work.take<double>(bl);
+    unsigned long long* val[2];
+    unsigned long long* key[2];
+    double* prof[2];
+    int64_t* idxin[2];
+    int64_t* idxout[2];
+    for (int s = 0; s < nseg; ++s) {
+        val[s] = c->work.take<unsigned long long>(span);
+        key[s] = c->work.take<unsigned long long>(span);
+        prof[s] = c->work.take<double>(span);
+        idxin[s] = c->work.take<int64_t>(span);
+        idxout[s] = c->work.take<int64_t>(span);
+    }
+    auto* counter = c->work.take<unsigned int>(1);
+    // A_hor: elements 0..tl-1+w are read; A_vert: 0..tl-2+w plus the element
+    // the reference reads one past at its very last update, which is replaced by 0.
+    MP_HIP(hipMemcpyAsync(dAh, A_hor, sizeof(double) * na, hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(dAv, A_vert, sizeof(double) * (na - 1), hipMemcpyHostToDevice, st));
+    MP_TRY(mp::launch_fill_u64(reinterpret_cast<unsigned long long*>(dAv + na - 1), 2, 0ull, st));
+    MP_HIP(hipMemcpyAsync(dsh, s_hor, sizeof(double) * tl, hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(dmh, mu_hor, sizeof(double) * tl, hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(dsv, s_vert, sizeof(double) * tl, hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(dmv, mu_vert, sizeof(double) * tl, hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(dq, tmpQ, sizeof(double) * bl, hipMemcpyHostToDevice, st));
+    const double* hprof[2] = {prof_colmin + lo_off, prof_rowmin};
+    const int64_t* hidx[2] = {idx_colmin + lo_off, idx_rowmin};
+    for (int s = 0; s < nseg; ++s) {
+        MP_HIP(hipMemcpyAsync(prof[s], hprof[s], sizeof(double) * span, hipMemcpyHostToDevice, st));
+        MP_HIP(hipMemcpyAsync(idxin[s], hidx[s], sizeof(int64_t) * span, hipMemcpyHostToDevice, st));
+        MP_TRY(mp::launch_pack_block(prof[s], static_cast<int>(span), ib, val[s], key[s], st));
+    }
+    MP_TRY(mp::launch_fill_u32(counter, 1, 0u, st));
+    mp::WalkF64Launch a{};
+    a.Ah = dAh;
+    a.sh = dsh;
+    a.mh = dmh;
+    a.Av = dAv;
+    a.sv = dsv;
+    a.mv = dmv;
+    a.qt = dq;
+    if (overlap) {
+        a.colval = val[0] - lo_off;
+        a.colkey = key[0] - lo_off;
+        a.rowval = val[0] - lo_off + pdiff;
+        a.rowkey = key[0] - lo_off + pdiff;
+    } else {
+        a.colval = val[0];
+        a.colkey = key[0];
+        a.rowval = val[1];
+        a.rowkey = key[1];
+    }
+    a.counter = counter;
+    a.row_off = basecol - pmin;
+    a.col_off = baserow - pmin;
+    a.blocklen = bl;
+    a.tl = tl;
+    a.m = static_cast<int>(w);
+    a.ib = ib;
+    a.writeback = 1;
+    a.band_k = 1;
+    const int nb = mp::walk_f64_nbands(bl, 1);
+    a.groups = std::max(1, std::min(((nb + 1) / 2 + 3) / 4, 8 * c->cus));
+    MP_TRY(mp::launch_walk_f64(a, st));
+    for (int s = 0; s < nseg; ++s) {
+        MP_TRY(mp::launch_unpack_block(val[s], key[s], static_cast<int>(span), ib, pmin, idxin[s],
+                                       prof[s], idxout[s], st));
+    }
+    double* oprof[2] = {prof_colmin + lo_off, prof_rowmin};
+    int64_t* oidx[2] = {idx_colmin + lo_off, idx_rowmin};
+    for (int s = 0; s < nseg; ++s) {
+        MP_HIP(hipMemcpyAsync(oprof[s], prof[s], sizeof(double) * span, hipMemcpyDeviceToHost, st));
+        MP_HIP(hipMemcpyAsync(oidx[s], idxout[s], sizeof(int64_t) * span, hipMemcpyDeviceToHost, st));
+    }
+    MP_HIP(hipMemcpyAsync(tmpQ, dq, sizeof(double) * bl, hipMemcpyDeviceToHost, st));
+    MP_HIP(hipStreamSynchronize(st));
+    return MP_OK;
+}
+
+}  // extern "C"

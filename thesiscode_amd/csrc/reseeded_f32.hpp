@@ -1,0 +1,43 @@
+// Descriptors shared by the host planner and the FP32-reseeded kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mp {
+
+// One self-join of a batch.  All per-series arrays (series A, mu, sigma, df,
+// dg, inv, keys, profile, index) live at element offset `off` of their
+// concatenated buffers.
+struct SeriesDesc {
+    int64_t off;       // element offset of this series in every per-series buffer
+    int n, m, P;       // series length, window, profile length
+    int base;          // first diagonal: exclusion + 1
+    int tl;            // diagonals = columns of the triangle: P - base
+    int nbands;        // bands of 64*K diagonals
+    int band_off;      // offset of this series' band prefix (nbands+1 entries)
+    float init_c1;     // 1 + rho of the initial score -1.0: 1 - 1/m
+    int pad;
+};
+
+struct F32rBatch {
+    const SeriesDesc* descs;    // device, nseries
+    const int64_t* entry_prefix;  // device, nseries+1: prefix of P
+    const int64_t* tile_prefix;   // device, nseries+1: prefix of tiles
+    const int64_t* band_prefix;   // device, concatenated per-series tile prefixes over bands
+    int nseries;
+    int64_t nentries, ntiles;
+    int band_k, tile_len;
+    const double* A;
+    double *mu, *sigma;
+    float *df, *dg, *inv;
+    unsigned long long* keys;
+    unsigned int* counter;
+};
+
+int launch_f32r_prep(const F32rBatch& b, hipStream_t st);
+int launch_f32r_tiles(const F32rBatch& b, int groups, hipStream_t st);
+int launch_f32r_unpack(const F32rBatch& b, int refine, int64_t index_init, double* profile,
+                       int64_t* index, hipStream_t st);
+
+}  // namespace mp
