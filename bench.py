@@ -81,6 +81,129 @@ def run_gpu(workload: str, steps: int, warmup: int, band_k: int, tile_len: int):
     }
 
 
+def batch_spec(seed: int = 3, count: int = 4096):
+    """4,096 FP32 random walks, lengths log-uniform in [2^10, 2^17], windows from {32,64,128,256}."""
+    import thesiscode_amd as tc
+
+    u = tc.uniforms(seed, count, stream=0)
+    v = tc.uniforms(seed, count, stream=1)
+    lengths = np.floor(2.0 ** (10.0 + 7.0 * u)).astype(np.int64)
+    windows = np.array([32, 64, 128, 256], dtype=np.int32)[np.minimum((4 * v).astype(int), 3)]
+    return lengths, windows
+
+
+def gen_batch_series(seed, lengths):
+    import concurrent.futures as cf
+
+    import thesiscode_amd as tc
+
+    def one(s):
+        return tc.random_walk(seed, int(lengths[s]), stream=100 + s).astype(np.float32).astype(np.float64)
+
+    with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(one, range(len(lengths))))
+
+
+def run_batch_bench(steps: int, warmup: int, tune: bool):
+    import thesiscode_amd as tc
+
+    lengths, windows = batch_spec()
+    series = gen_batch_series(3, lengths)
+    with tc.Device(0) as dev:
+        with tc.Batch(dev, lengths, windows) as b:
+            d_ts = dev.upload(np.concatenate(series))
+            P = int(b.out_offsets[-1])
+            d_p = dev.alloc(8 * P)
+            d_i = dev.alloc(8 * P)
+            tuned = b.tune(d_ts, d_p, d_i, max_tile_len=4096) if tune else None
+            for _ in range(warmup):
+                b.run(d_ts, d_p, d_i, use_graph=True)
+            dev.sync()
+            t0 = time.perf_counter()
+            dev_ms = []
+            for _ in range(steps):
+                dev_ms.append(b.run(d_ts, d_p, d_i, use_graph=True)["total_ms"])
+            wall = time.perf_counter() - t0
+            info = b.info()
+    return {"cells": info["cells"], "wall_s": wall, "device_ms": float(np.mean(dev_ms)), "band_k": info["band_k"],
+            "tile_len": info["tile_len"], "series": len(lengths), "points": int(lengths.sum()),
+            "tiles": info["tiles"], "tuned": tuned}
+
+
+def serve_spec(seed: int, seconds: float, rate: float):
+    """Poisson arrivals; lognormal lengths (median 2^14, sigma 1, clipped [2^10, 2^19]); m in {64,...,512}."""
+    import thesiscode_amd as tc
+
+    count = int(rate * seconds * 1.2) + 100
+    u = tc.uniforms(seed, count, stream=0)
+    arrivals = np.cumsum(-np.log1p(-u) / rate)
+    count = int(np.searchsorted(arrivals, seconds))
+    arrivals = arrivals[:count]
+    z = tc.normals(seed, count, stream=1)
+    lengths = np.clip(np.floor(np.exp(np.log(2.0 ** 14) + z)), 2 ** 10, 2 ** 19).astype(np.int64)
+    windows = np.array([64, 128, 256, 512])[np.minimum((4 * tc.uniforms(seed, count, stream=2)).astype(int), 3)]
+    return arrivals, lengths, windows
+
+
+def run_serve_bench(seconds: float, rate: float, budget_gb: float, tick_cells: float):
+    """Requests are slices of one long random walk (seed 5) at random offsets."""
+    import threading
+
+    import thesiscode_amd as tc
+
+    arrivals, lengths, windows = serve_spec(5, seconds, rate)
+    walk = tc.random_walk(5, 1 << 25, stream=3)
+    offs = np.floor(tc.uniforms(5, len(lengths), stream=4) * (walk.size - lengths)).astype(np.int64)
+    cells = np.array([tc.cells(int(n), int(m)) for n, m in zip(lengths, windows)], dtype=np.float64)
+    lat = {}
+    queue = {}
+    with tc.Device(0) as dev:
+        with tc.Server(dev, hbm_budget_bytes=int(budget_gb * (1 << 30)), tick_cells=int(tick_cells)) as srv:
+            # warm the kernels
+            srv.wait(srv.submit(walk[:20000], 128))
+            ids = {}
+            t0 = time.perf_counter()
+
+            def submitter():
+                for k in range(len(lengths)):
+                    delay = t0 + arrivals[k] - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                    rid = srv.submit(walk[offs[k]:offs[k] + lengths[k]], int(windows[k]))
+                    ids[rid] = k
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            last_print = t0
+            finished = {}
+            deadline = t0 + seconds + 300.0
+            while th.is_alive() or len(lat) < len(lengths):
+                for rid, st, l_ms, q_ms, _, _ in srv.completed():
+                    if st != 0:
+                        raise RuntimeError(f"request failed with status {st}")
+                    finished[rid] = (l_ms, q_ms)
+                for rid in [r for r in finished if r in ids]:
+                    lat[ids[rid]], queue[ids[rid]] = finished.pop(rid)
+                time.sleep(0.002)
+                now = time.perf_counter()
+                if now > deadline:
+                    raise RuntimeError(f"serving did not finish: {len(lat)}/{len(lengths)} done")
+                if now - last_print > 10:
+                    print(f"serve: {now - t0:.0f} s, {len(lat)}/{len(lengths)} done", file=sys.stderr, flush=True)
+                    last_print = now
+            t_end = time.perf_counter()
+            th.join()
+            st = srv.stats()
+    lat_v = np.array([lat[k] for k in range(len(lengths))])
+    q_v = np.array([queue[k] for k in range(len(lengths))])
+    return {"requests": len(lengths), "offered_rate_per_s": rate, "duration_s": seconds,
+            "cells": float(cells.sum()), "offered_cells_per_s": float(cells.sum() / seconds),
+            "served_cells_per_s": float(cells.sum() / (t_end - t0)), "wall_s": t_end - t0,
+            "p50_ms": float(np.percentile(lat_v, 50)), "p99_ms": float(np.percentile(lat_v, 99)),
+            "mean_ms": float(lat_v.mean()), "max_ms": float(lat_v.max()),
+            "queue_p99_ms": float(np.percentile(q_v, 99)), "ticks": st["ticks"]}
+
+
 def cpu_baseline(workload: str, target_s: float = 12.0):
     """Oracle restatement of scrimp_triv_par on a bounded sample (every k-th block)."""
     from oracle import oracle as orc
@@ -118,6 +241,13 @@ def main() -> int:
     ap.add_argument("--band-k", type=int, default=0)
     ap.add_argument("--tile-len", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", action="store_true", help="run the 4,096-series batched workload instead")
+    ap.add_argument("--tune", action="store_true", help="autotune the batch before timing it")
+    ap.add_argument("--serve", action="store_true", help="run the Poisson serving workload instead")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=16.0)
+    ap.add_argument("--serve-tick-cells", type=float, default=4e9)
     ap.add_argument("--no-variant", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -125,6 +255,22 @@ def main() -> int:
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.batch:
+        r = run_batch_bench(args.steps, args.warmup, args.tune)
+        ms = 1000.0 * r["wall_s"] / args.steps
+        print(json.dumps({"metric": "matrix-profile cells/s", "value": r["cells"] / (ms / 1000.0), "unit": "cells/s",
+                          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+                          "higher_is_better": True, "dtype": "f32 recurrence, f64 stats/seeds",
+                          "data": "synthetic FP32 random walks", "config": {"workload": "batch-4096", **{
+                              k: v for k, v in r.items() if k not in ("wall_s",)}}}), flush=True)
+        return 0
+    if args.serve:
+        r = run_serve_bench(args.serve_seconds, args.serve_rate, args.serve_budget_gb, args.serve_tick_cells)
+        print(json.dumps({"metric": "matrix-profile cells/s; p50/p99 latency", "value": r["served_cells_per_s"],
+                          "unit": "cells/s", "n_gpus": 1, "higher_is_better": True,
+                          "dtype": "f32 recurrence, f64 stats/seeds", "data": "synthetic random-walk requests",
+                          "config": {"workload": "serve-poisson", **r}}), flush=True)
+        return 0
     r = run_gpu(args.workload, args.steps, args.warmup, args.band_k, args.tile_len)
     ms_per_step = 1000.0 * r["wall_s"] / args.steps
     value = r["cells"] / (ms_per_step / 1000.0)

@@ -127,6 +127,58 @@ int mp_eval_diag_block_triangle_f64(double prof_colmin[], int64_t idx_colmin[],
                                     const double mu_vert[], int64_t baserow,
                                     int64_t basecol);
 
+/* ---- batched FP32-reseeded self-joins --------------------------------------
+ * Thousands of independent series of uneven length and window in one
+ * persistent launch.  Series s occupies lengths[s] consecutive doubles of the
+ * concatenated input and P_s = lengths[s]-windows[s]+1 consecutive entries of
+ * the concatenated profile/index outputs (device pointers).  The exclusion of
+ * each series is windows[s]/4 unless opts->exclusion >= 0.  With use_graph
+ * the whole batch (statistics, tiles, unpack) is captured once in a hipGraph
+ * and replayed while the pointers stay the same.  mp_batch_tune times band
+ * widths 4/8 and tile lengths 256..max_tile_len on the real batch, keeps the
+ * fastest and records it in an on-disk cache that later mp_batch_create calls
+ * consult ($SCRIMP_HIP_TUNE_CACHE, default ~/.cache/scrimp_hip/).
+ * No counterpart in the reference, which joins one series per run.        */
+typedef struct mp_batch mp_batch;
+int mp_batch_create(mp_ctx* ctx, int32_t nseries, const int64_t* lengths,
+                    const int32_t* windows, const mp_opts* opts, mp_batch** out);
+int mp_batch_run(mp_batch* batch, const double* d_ts, double* d_profile, int64_t* d_index,
+                 int32_t use_graph, mp_timing* timing);
+int mp_batch_tune(mp_batch* batch, const double* d_ts, double* d_profile, int64_t* d_index,
+                  int32_t max_tile_len, mp_timing* best);
+int mp_batch_info(const mp_batch* batch, int64_t* total_n, int64_t* total_p, int64_t* ntiles,
+                  double* cells, int32_t* band_k, int32_t* tile_len);
+int mp_batch_destroy(mp_batch* batch);
+
+/* ---- serving: continuous batching of self-join requests --------------------
+ * mp_server_create starts an engine thread on the context's device with an
+ * HBM budget for resident request state (60 bytes per point: series,
+ * statistics, recurrence terms, partial-profile keys, outputs).  Requests are
+ * admitted in arrival order while they fit; each engine tick launches the
+ * FP32-reseeded tile kernel once over a share of the remaining tiles of every
+ * admitted request (about tick_cells cells per tick; <= 0 picks 4e9), so
+ * requests that arrive while others run join the next launch.  A request's
+ * profile/index (P = n-m+1 entries each, host memory the caller keeps alive
+ * until completion) are written when it completes.  opts: exclusion, band_k,
+ * tile_len (re-seed interval), refine, index_init as for mp_selfjoin_f32r.
+ * submit returns MP_ERR_ADMISSION when a request can never fit the budget.
+ * poll/wait return the request's own status and, once done, its latency
+ * (submit -> results written) and forget it.  No counterpart in the
+ * reference (a batch program that joins one series per run).              */
+typedef struct mp_server mp_server;
+int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
+                     const mp_opts* opts, mp_server** out);
+int mp_server_submit(mp_server* server, const double* ts, int64_t n, int32_t m,
+                     double* profile_out, int64_t* index_out, uint64_t* request_id);
+int mp_server_poll(mp_server* server, uint64_t request_id, int32_t* done, double* latency_ms,
+                   double* queue_ms);
+int mp_server_wait(mp_server* server, uint64_t request_id, double* latency_ms);
+int mp_server_completed(mp_server* server, int32_t max, uint64_t* ids, int32_t* statuses,
+                        double* latency_ms, double* queue_ms, int32_t* count);
+int mp_server_stats(mp_server* server, int64_t* ticks, int64_t* completed, double* cells_done,
+                    int64_t* queued, int64_t* active, int64_t* hbm_used_bytes);
+int mp_server_destroy(mp_server* server);
+
 /* ---- synthetic inputs (CPU only, bit-identical on every host) -------------
  * Counter-based Philox4x32-10 (Salmon et al., SC'11; key = 64-bit seed,
  * words {seed lo, seed hi}) + Box-Muller.  None of these has a counterpart in

@@ -19,7 +19,7 @@ from ._ffi import Opts, ScrimpError, Timing, check, lib
 __all__ = [
     "ScrimpError", "Device", "selfjoin_f64", "selfjoin_f32r", "eval_diag_block_triangle_f64",
     "random_walk", "normals", "uniforms", "precision_series", "profile_length", "exclusion_zone",
-    "cells", "ALGORITHMS", "make_opts",
+    "cells", "ALGORITHMS", "make_opts", "Batch", "Server",
 ]
 
 
@@ -148,6 +148,135 @@ class Device:
             self._bufs = []
             lib().mp_ctx_destroy(self._ctx)
             self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """Many FP32-reseeded self-joins in one persistent launch (optionally a replayed hipGraph).
+
+    The series are concatenated back to back in device memory; outputs are
+    concatenated over the profile lengths n_s - m_s + 1.
+    """
+
+    def __init__(self, dev: Device, lengths, windows, *, exclusion: int = -1, band_k: int = 0,
+                 tile_len: int = 0, refine: bool = True, index_init: int = -1):
+        self.lengths = np.ascontiguousarray(lengths, dtype=np.int64)
+        self.windows = np.ascontiguousarray(windows, dtype=np.int32)
+        self._h = C.c_void_p()
+        o = make_opts(exclusion, band_k, tile_len, refine, index_init)
+        check(lib().mp_batch_create(dev._ctx, len(self.lengths), _lptr(self.lengths),
+                                    self.windows.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(o),
+                                    C.byref(self._h)), "mp_batch_create")
+        self.profile_lengths = self.lengths - self.windows + 1
+        self.offsets = np.concatenate([[0], np.cumsum(self.lengths)])
+        self.out_offsets = np.concatenate([[0], np.cumsum(self.profile_lengths)])
+
+    def info(self) -> dict:
+        tn, tp, nt = C.c_int64(), C.c_int64(), C.c_int64()
+        cl = C.c_double()
+        bk, tl = C.c_int32(), C.c_int32()
+        check(lib().mp_batch_info(self._h, C.byref(tn), C.byref(tp), C.byref(nt), C.byref(cl), C.byref(bk),
+                                  C.byref(tl)), "mp_batch_info")
+        return {"total_n": tn.value, "total_p": tp.value, "tiles": nt.value, "cells": cl.value,
+                "band_k": bk.value, "tile_len": tl.value}
+
+    def run(self, d_ts: int, d_prof: int, d_idx: int, use_graph: bool = True) -> dict:
+        tm = Timing()
+        check(lib().mp_batch_run(self._h, C.c_void_p(d_ts), C.c_void_p(d_prof), C.c_void_p(d_idx),
+                                 1 if use_graph else 0, C.byref(tm)), "mp_batch_run")
+        return tm.as_dict()
+
+    def tune(self, d_ts: int, d_prof: int, d_idx: int, max_tile_len: int = 4096) -> dict:
+        tm = Timing()
+        check(lib().mp_batch_tune(self._h, C.c_void_p(d_ts), C.c_void_p(d_prof), C.c_void_p(d_idx),
+                                  max_tile_len, C.byref(tm)), "mp_batch_tune")
+        return tm.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            lib().mp_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """Continuous-batching request server on one GPU (FP32-reseeded self-joins)."""
+
+    def __init__(self, dev: Device, hbm_budget_bytes: int = 8 << 30, tick_cells: int = 0, *,
+                 exclusion: int = -1, band_k: int = 0, tile_len: int = 0, refine: bool = True,
+                 index_init: int = -1):
+        self._h = C.c_void_p()
+        o = make_opts(exclusion, band_k, tile_len, refine, index_init)
+        check(lib().mp_server_create(dev._ctx, hbm_budget_bytes, tick_cells, C.byref(o), C.byref(self._h)),
+              "mp_server_create")
+        self._keep: dict[int, tuple] = {}
+
+    def submit(self, ts, m: int) -> int:
+        ts = np.ascontiguousarray(ts, dtype=np.float64)
+        P = ts.size - m + 1
+        prof = np.empty(max(P, 0))
+        idx = np.empty(max(P, 0), dtype=np.int64)
+        rid = C.c_uint64()
+        check(lib().mp_server_submit(self._h, _dptr(ts), ts.size, m, _dptr(prof), _lptr(idx), C.byref(rid)),
+              "mp_server_submit")
+        self._keep[rid.value] = (prof, idx)
+        return rid.value
+
+    def poll(self, rid: int):
+        """None while running, else (profile, index, latency_ms, queue_ms)."""
+        done = C.c_int32()
+        lat = C.c_double()
+        q = C.c_double()
+        rc = lib().mp_server_poll(self._h, rid, C.byref(done), C.byref(lat), C.byref(q))
+        if rc == 0 and not done.value:
+            return None
+        prof, idx = self._keep.pop(rid)
+        check(rc, "mp_server_poll")
+        return prof, idx, lat.value, q.value
+
+    def wait(self, rid: int):
+        lat = C.c_double()
+        rc = lib().mp_server_wait(self._h, rid, C.byref(lat))
+        prof, idx = self._keep.pop(rid)
+        check(rc, "mp_server_wait")
+        return prof, idx, lat.value
+
+    def completed(self, max_items: int = 4096):
+        """Drain finished requests: list of (id, status, latency_ms, queue_ms, profile, index)."""
+        ids = (C.c_uint64 * max_items)()
+        st = (C.c_int32 * max_items)()
+        lat = (C.c_double * max_items)()
+        q = (C.c_double * max_items)()
+        cnt = C.c_int32()
+        check(lib().mp_server_completed(self._h, max_items, ids, st, lat, q, C.byref(cnt)), "mp_server_completed")
+        out = []
+        for k in range(cnt.value):
+            prof, idx = self._keep.pop(ids[k])
+            out.append((ids[k], st[k], lat[k], q[k], prof, idx))
+        return out
+
+    def stats(self) -> dict:
+        t, c, q, a, h = (C.c_int64() for _ in range(5))
+        cd = C.c_double()
+        check(lib().mp_server_stats(self._h, C.byref(t), C.byref(c), C.byref(cd), C.byref(q), C.byref(a),
+                                    C.byref(h)), "mp_server_stats")
+        return {"ticks": t.value, "completed": c.value, "cells_done": cd.value, "queued": q.value,
+                "active": a.value, "hbm_used_bytes": h.value}
+
+    def close(self) -> None:
+        if self._h:
+            lib().mp_server_destroy(self._h)
+            self._h = C.c_void_p()
 
     def __enter__(self):
         return self

@@ -63,6 +63,19 @@ SIGNATURES = {
     "mp_selfjoin_f32r_dev": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P, C.POINTER(Opts), C.POINTER(Timing)]),
     "mp_eval_diag_block_triangle_f64": (C.c_int, [_D, _L, _D, _L, _D, C.c_int64, C.c_int64, _D, _D,
                                                   C.c_int64, _D, _D, _D, _D, C.c_int64, C.c_int64]),
+    "mp_batch_create": (C.c_int, [_P, C.c_int32, _L, C.POINTER(C.c_int32), C.POINTER(Opts), C.POINTER(_P)]),
+    "mp_batch_run": (C.c_int, [_P, _P, _P, _P, C.c_int32, C.POINTER(Timing)]),
+    "mp_batch_tune": (C.c_int, [_P, _P, _P, _P, C.c_int32, C.POINTER(Timing)]),
+    "mp_batch_info": (C.c_int, [_P, _L, _L, _L, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "mp_batch_destroy": (C.c_int, [_P]),
+    "mp_server_create": (C.c_int, [_P, C.c_uint64, C.c_int64, C.POINTER(Opts), C.POINTER(_P)]),
+    "mp_server_submit": (C.c_int, [_P, _D, C.c_int64, C.c_int32, _D, _L, C.POINTER(C.c_uint64)]),
+    "mp_server_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mp_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_double)]),
+    "mp_server_completed": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "mp_server_stats": (C.c_int, [_P, _L, _L, C.POINTER(C.c_double), _L, _L, _L]),
+    "mp_server_destroy": (C.c_int, [_P]),
     "mp_gen_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint32)]),
     "mp_gen_normals": (None, [C.c_uint64, C.c_uint32, C.c_int64, C.c_int64, _D]),
     "mp_gen_uniforms": (None, [C.c_uint64, C.c_uint32, C.c_int64, C.c_int64, _D]),

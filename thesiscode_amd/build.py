@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["exact_f64.hip", "reseeded_f32.hip", "api.cpp", "datagen.cpp", "serve.cpp"]
+SOURCES = ["exact_f64.hip", "reseeded_f32.hip", "api.cpp", "batch.cpp", "datagen.cpp", "serve.cpp"]
 
 
 def _hipcc() -> str:
@@ -63,6 +63,33 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+HOST = CSRC / "host"
+TOOLS = {
+    "scrimppp_hip": ["scrimppp_hip.cpp", "scrimp_hip_algos.cpp", "profile_io.cpp"],
+    "cvt_ts_hip": ["cvt_tools.cpp", "profile_io.cpp"],
+    "cvt_matprof_hip": ["cvt_tools.cpp", "profile_io.cpp"],
+}
+
+
+def build_tools(verbose: bool = False) -> list[Path]:
+    """Host executables (CLI + converters) on the C ABI, linked against libscrimp_hip.so."""
+    gxx = shutil.which("g++") or "g++"
+    outs = []
+    for name, srcs in TOOLS.items():
+        exe = PKG / name
+        cmd = [gxx, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", f"-I{HOST}",
+               *[str(HOST / s) for s in srcs], f"-L{PKG}", "-lscrimp_hip", "-Wl,-rpath,$ORIGIN", "-o", str(exe)]
+        if name.startswith("cvt_"):
+            cmd = [c for c in cmd if c not in ("-lscrimp_hip", f"-L{PKG}")]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"tool build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        outs.append(exe)
+        if verbose:
+            print(f"built {exe}")
+    return outs
+
+
 def build_oracle(verbose: bool = False) -> Path:
     r = subprocess.run(["make", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
     if r.returncode != 0:
@@ -75,5 +102,6 @@ def build_oracle(verbose: bool = False) -> Path:
 
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_tools(verbose=True)
     if "--oracle" in sys.argv:
         build_oracle(verbose=True)

@@ -11,6 +11,7 @@
 #include <new>
 #include <vector>
 
+#include "ctx.hpp"
 #include "reseeded_f32.hpp"
 #include "scrimp_hip.h"
 
@@ -51,49 +52,8 @@ int launch_walk_f64(const WalkF64Launch& a, hipStream_t st);
         if ((expr) != hipSuccess) return MP_ERR_DEVICE; \
     } while (0)
 
-namespace {
-
-constexpr int64_t kMaxProfile = int64_t(1) << 30;
-
-// Grow-only device arena owned by a context; sub-allocations are 256-byte aligned.
-struct Arena {
-    char* base = nullptr;
-    size_t cap = 0;
-    size_t used = 0;
-    int reserve(size_t bytes) {
-        if (bytes <= cap) return MP_OK;
-        if (base) (void)hipFree(base);
-        base = nullptr;
-        cap = 0;
-        if (hipMalloc(&base, bytes) != hipSuccess) {
-            base = nullptr;
-            return MP_ERR_OOM;
-        }
-        cap = bytes;
-        return MP_OK;
-    }
-    template <class T>
-    T* take(size_t count) {
-        const size_t b = (count * sizeof(T) + 255) & ~size_t(255);
-        T* p = reinterpret_cast<T*>(base + used);
-        used += b;
-        return p;
-    }
-};
-
-size_t aligned_bytes(size_t count, size_t elem) { return (count * elem + 255) & ~size_t(255); }
-
-}  // namespace
-
-struct mp_ctx {
-    int device = 0;
-    int cus = 256;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {};
-    Arena work;   // per-call scratch
-    Arena io;     // staging for host-pointer calls
-    std::mutex lock;
-};
+using mp::aligned_bytes;
+using mp::kMaxProfile;
 
 namespace {
 
@@ -447,6 +407,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     const int64_t ntiles = bprefix[nbands];
     mp::SeriesDesc desc{};
     desc.off = 0;
+    desc.out_off = 0;
     desc.n = static_cast<int>(n);
     desc.m = m;
     desc.P = static_cast<int>(g.P);

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32(
         if (tile >= ntiles) break;
         const int s = find_series(tile_prefix, nseries, tile);
         const SeriesDesc d = descs[s];
-        const int64_t u = tile - tile_prefix[s];
+        const int64_t u = tile - tile_prefix[s] + d.tile_base;
         const int64_t* bp = band_prefix + d.band_off;
         const int b = find_series(bp, d.nbands, u);
         const int q = static_cast<int>(u - bp[b]);
@@ -356,8 +356,8 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
             }
         }
         if (lane == 0) {
-            profile[d.off + i] = d2;
-            index[d.off + i] = idx;
+            profile[d.out_off + i] = d2;
+            index[d.out_off + i] = idx;
         }
     }
 }

@@ -11,6 +11,8 @@ namespace mp {
 // concatenated buffers.
 struct SeriesDesc {
     int64_t off;       // element offset of this series in every per-series buffer
+    int64_t out_off;   // offset of its P entries in the profile/index outputs
+    int64_t tile_base; // first tile of this series covered by this descriptor (a slice of its tiles)
     int n, m, P;       // series length, window, profile length
     int base;          // first diagonal: exclusion + 1
     int tl;            // diagonals = columns of the triangle: P - base
