@@ -51,7 +51,9 @@ typedef struct mp_opts {
     int32_t index_init;  /* index reported when nothing beat the initial score: -1 (the
                             parallel drivers) or 0 (scrimp_vert_blocked,
                             ScrimpVertBlocked.cpp:104)                                     */
-    int32_t reserved[3];
+    int32_t accum64;     /* FP32 path: 0 = FP32 accumulation of the covariance, 1 = the same
+                            FP32 terms accumulated in FP64                                  */
+    int32_t reserved[2];
 } mp_opts;
 
 void mp_opts_default(mp_opts* o);

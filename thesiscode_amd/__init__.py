@@ -47,7 +47,7 @@ def cells(n: int, m: int, excl: int | None = None) -> int:
 
 
 def make_opts(exclusion: int = -1, band_k: int = 0, tile_len: int = 0, refine: bool = True,
-              index_init: int = -1) -> Opts:
+              index_init: int = -1, accum64: bool = False) -> Opts:
     o = Opts()
     lib().mp_opts_default(C.byref(o))
     o.exclusion = exclusion
@@ -55,6 +55,7 @@ def make_opts(exclusion: int = -1, band_k: int = 0, tile_len: int = 0, refine: b
     o.tile_len = tile_len
     o.refine = 1 if refine else 0
     o.index_init = index_init
+    o.accum64 = 1 if accum64 else 0
     return o
 
 
@@ -76,9 +77,12 @@ def selfjoin_f64(ts, m: int, *, exclusion: int = -1, band_k: int = 0, index_init
 
 
 def selfjoin_f32r(ts, m: int, *, exclusion: int = -1, band_k: int = 0, tile_len: int = 0,
-                  refine: bool = True, index_init: int = -1):
-    """FP32 recurrence, FP64 statistics and per-tile FP64 seeds (tile_len = re-seed interval)."""
-    return _selfjoin("mp_selfjoin_f32r", ts, m, make_opts(exclusion, band_k, tile_len, refine, index_init))
+                  refine: bool = True, index_init: int = -1, accum64: bool = False):
+    """FP32 recurrence, FP64 statistics and per-tile FP64 seeds (tile_len = re-seed interval).
+
+    accum64 keeps the FP32 terms but accumulates the covariance in FP64.
+    """
+    return _selfjoin("mp_selfjoin_f32r", ts, m, make_opts(exclusion, band_k, tile_len, refine, index_init, accum64))
 
 
 def eval_diag_block_triangle_f64(prof_colmin, idx_colmin, prof_rowmin, idx_rowmin, tmpQ, blocklen,
@@ -164,11 +168,11 @@ class Batch:
     """
 
     def __init__(self, dev: Device, lengths, windows, *, exclusion: int = -1, band_k: int = 0,
-                 tile_len: int = 0, refine: bool = True, index_init: int = -1):
+                 tile_len: int = 0, refine: bool = True, index_init: int = -1, accum64: bool = False):
         self.lengths = np.ascontiguousarray(lengths, dtype=np.int64)
         self.windows = np.ascontiguousarray(windows, dtype=np.int32)
         self._h = C.c_void_p()
-        o = make_opts(exclusion, band_k, tile_len, refine, index_init)
+        o = make_opts(exclusion, band_k, tile_len, refine, index_init, accum64)
         check(lib().mp_batch_create(dev._ctx, len(self.lengths), _lptr(self.lengths),
                                     self.windows.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(o),
                                     C.byref(self._h)), "mp_batch_create")
@@ -214,9 +218,9 @@ class Server:
 
     def __init__(self, dev: Device, hbm_budget_bytes: int = 8 << 30, tick_cells: int = 0, *,
                  exclusion: int = -1, band_k: int = 0, tile_len: int = 0, refine: bool = True,
-                 index_init: int = -1):
+                 index_init: int = -1, accum64: bool = False):
         self._h = C.c_void_p()
-        o = make_opts(exclusion, band_k, tile_len, refine, index_init)
+        o = make_opts(exclusion, band_k, tile_len, refine, index_init, accum64)
         check(lib().mp_server_create(dev._ctx, hbm_budget_bytes, tick_cells, C.byref(o), C.byref(self._h)),
               "mp_server_create")
         self._keep: dict[int, tuple] = {}

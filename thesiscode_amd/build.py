@@ -24,7 +24,7 @@ ARCH = os.environ.get("SCRIMP_HIP_ARCH", "gfx950")
 # -ffp-contract=off: the exact path's arithmetic is written with explicit fma()
 # and must not be re-fused.  Packed FP32 instructions are disabled.
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-command-line-argument",
 ]

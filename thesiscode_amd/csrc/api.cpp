@@ -169,6 +169,7 @@ void mp_opts_default(mp_opts* o) {
     o->exclusion = -1;
     o->refine = 1;
     o->index_init = -1;
+    o->accum64 = 0;
 }
 
 int mp_device_count(int* count) {
@@ -396,6 +397,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     const int T = default_tile_len(m, o ? o->tile_len : 0);
     if (T % 16 != 0 || T <= 0) return MP_ERR_BAD_ARG;
     const int refine = o ? (o->refine != 0) : 1;
+    const int accum64 = o ? (o->accum64 != 0) : 0;
     const int64_t index_init = (o && o->index_init == 0) ? 0 : -1;
     const int BW = 64 * band_k;
     const int nbands = (g.W + BW - 1) / BW;
@@ -424,6 +426,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     hipStream_t st = c->stream;
     const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n + 64, 8) +
                         3 * aligned_bytes(n + 64, 4) + aligned_bytes(n + 64, 8) +
+                        (accum64 ? 2 * aligned_bytes(n + 64, 8) : 0) +
                         aligned_bytes(1, sizeof(mp::SeriesDesc)) + 2 * aligned_bytes(2, 8) +
                         aligned_bytes(nbands + 1, 8) + aligned_bytes(1, 4);
     MP_TRY(c->work.reserve(need));
@@ -435,6 +438,8 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     float* dg = c->work.take<float>(n + 64);
     float* inv = c->work.take<float>(n + 64);
     auto* keys = c->work.take<unsigned long long>(n + 64);
+    double* df64 = accum64 ? c->work.take<double>(n + 64) : nullptr;
+    double* dg64 = accum64 ? c->work.take<double>(n + 64) : nullptr;
     auto* ddesc = c->work.take<mp::SeriesDesc>(1);
     auto* dep = c->work.take<int64_t>(2);
     auto* dtp = c->work.take<int64_t>(2);
@@ -465,6 +470,9 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     b.df = df;
     b.dg = dg;
     b.inv = inv;
+    b.df64 = df64;
+    b.dg64 = dg64;
+    b.accum64 = accum64;
     b.keys = keys;
     b.counter = counter;
     MP_TRY(mp::launch_f32r_prep(b, st));

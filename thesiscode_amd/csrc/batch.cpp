@@ -109,6 +109,7 @@ struct mp_batch {
     hipEvent_t ev[3] = {};
     mp::BatchPlan plan;
     int refine = 1;
+    int accum64 = 0;
     int64_t index_init = -1;
     int excl = -1;
     std::vector<int64_t> lengths;
@@ -131,7 +132,8 @@ namespace {
 int upload_plan(mp_batch* b) {
     const mp::BatchPlan& p = b->plan;
     const int64_t N = p.total_n + 64;
-    const size_t need = 2 * al(N, 8) + 3 * al(N, 4) + al(N, 8) + al(p.nseries, sizeof(mp::SeriesDesc)) +
+    const size_t need = 2 * al(N, 8) + 3 * al(N, 4) + al(N, 8) + (b->accum64 ? 2 * al(N, 8) : 0) +
+                        al(p.nseries, sizeof(mp::SeriesDesc)) +
                         2 * al(p.nseries + 1, 8) + al(p.bprefix.size(), 8) + al(1, 4);
     if (need > b->bytes) {
         if (b->mem) (void)hipFree(b->mem);
@@ -153,6 +155,9 @@ int upload_plan(mp_batch* b) {
     d.dg = reinterpret_cast<float*>(take(al(N, 4)));
     d.inv = reinterpret_cast<float*>(take(al(N, 4)));
     d.keys = reinterpret_cast<unsigned long long*>(take(al(N, 8)));
+    d.df64 = b->accum64 ? reinterpret_cast<double*>(take(al(N, 8))) : nullptr;
+    d.dg64 = b->accum64 ? reinterpret_cast<double*>(take(al(N, 8))) : nullptr;
+    d.accum64 = b->accum64;
     auto* dd = reinterpret_cast<mp::SeriesDesc*>(take(al(p.nseries, sizeof(mp::SeriesDesc))));
     auto* dep = reinterpret_cast<int64_t*>(take(al(p.nseries + 1, 8)));
     auto* dtp = reinterpret_cast<int64_t*>(take(al(p.nseries + 1, 8)));
@@ -220,7 +225,7 @@ std::string tune_key(const mp_batch* b) {
     const double lg_cells = std::floor(std::log2(std::max(1.0, b->plan.cells)));
     const double lg_n = std::floor(std::log2(std::max<double>(1.0, static_cast<double>(b->plan.total_n) / b->plan.nseries)));
     std::ostringstream s;
-    s << arch << "|m" << med_m << "|lc" << lg_cells << "|ln" << lg_n << "|s" << (b->plan.nseries > 1 ? "batch" : "one");
+    s << arch << (b->accum64 ? "|a64" : "|a32") << "|m" << med_m << "|lc" << lg_cells << "|ln" << lg_n << "|s" << (b->plan.nseries > 1 ? "batch" : "one");
     std::string k = s.str();
     for (auto& ch : k)
         if (ch == ' ') ch = '_';
@@ -285,6 +290,7 @@ int mp_batch_create(mp_ctx* ctx, int32_t nseries, const int64_t* lengths, const 
     b->lengths.assign(lengths, lengths + nseries);
     b->windows.assign(windows, windows + nseries);
     b->refine = o ? (o->refine != 0) : 1;
+    b->accum64 = o ? (o->accum64 != 0) : 0;
     b->index_init = (o && o->index_init == 0) ? 0 : -1;
     b->excl = o ? o->exclusion : -1;
     if (o && o->exclusion < -1) return fail(MP_ERR_BAD_WINDOW);

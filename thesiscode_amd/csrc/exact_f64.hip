@@ -22,63 +22,30 @@
 // HBM with two 64-bit atomicMax per result: one on the exact ordered score,
 // one on a key of (truncated score, partner index).
 #include <type_traits>
+#include <utility>
 
 #include "common.hpp"
 
 namespace mp {
 
-// One wave scans the whole series in the reference's sequential order.  The
-// 64 lanes stage 1024-element chunks into LDS (double-buffered); lane 0 runs
-// the dependent FP64 chains.
+// One lane scans the whole series in the reference's sequential order (the
+// dependent FP64 add chain is the critical path; the loads are uniform and go
+// through the scalar cache, the stores are single-lane vector stores).
 __global__ __launch_bounds__(64) void k_ref_cumsum(const double* __restrict__ A, int n,
                                                   double* __restrict__ cs,
                                                   double* __restrict__ sq) {
-    constexpr int C = 1024;
-    __shared__ double in[2][C];
-    __shared__ double oc[C];
-    __shared__ double os[C];
-    const int lane = threadIdx.x;
-    for (int t = lane; t < C; t += 64) in[0][t] = (t < n) ? A[t] : 0.0;
-    __syncthreads();
-    double c = 0.0, s = 0.0;
-    const int nch = (n + C - 1) / C;
-    for (int ch = 0; ch < nch; ++ch) {
-        const int cur = ch & 1;
-        const int base = ch * C;
-        double pre[C / 64];
-#pragma unroll
-        for (int u = 0; u < C / 64; ++u) {
-            const int idx = base + C + lane + 64 * u;
-            pre[u] = (idx < n) ? A[idx] : 0.0;
-        }
-        const int len = min(C, n - base);
-        if (lane == 0) {
-            int t = 0;
-            if (ch == 0) {
-                const double a = in[cur][0];
-                c = a;
-                s = a * a;
-                oc[0] = c;
-                os[0] = s;
-                t = 1;
-            }
-#pragma unroll 8
-            for (; t < len; ++t) {
-                const double a = in[cur][t];
-                c = a + c;
-                s = fma(a, a, s);
-                oc[t] = c;
-                os[t] = s;
-            }
-        }
-        __syncthreads();
-        for (int t = lane; t < len; t += 64) {
-            cs[base + t] = oc[t];
-            sq[base + t] = os[t];
-        }
-#pragma unroll
-        for (int u = 0; u < C / 64; ++u) in[cur ^ 1][lane + 64 * u] = pre[u];
-        __syncthreads();
+    if (threadIdx.x != 0 || n <= 0) return;
+    double c = A[0];
+    double s = A[0] * A[0];
+    cs[0] = c;
+    sq[0] = s;
+#pragma unroll 16
+    for (int i = 1; i < n; ++i) {
+        const double a = A[i];
+        c = a + c;
+        s = fma(a, a, s);
+        cs[i] = c;
+        sq[i] = s;
     }
 }
 
@@ -153,24 +120,53 @@ __global__ void k_pack_block(const double* __restrict__ prof, int count, int ib,
     key[i] = o | ((1ull << ib) - 1ull);
 }
 
-template <int K>
+// (value, local diagonal id) merge: the larger score wins; exact ties keep the
+// larger id so that merging is symmetric and deterministic.  NaN never wins.
+struct ScoreId {
+    double v;
+    int id;
+};
+__device__ __forceinline__ ScoreId better(ScoreId a, ScoreId b) {
+    const bool take = b.v > a.v || (b.v == a.v && b.id > a.id);
+    return take ? b : a;
+}
+// strict version for the per-step tree (fixed operand order keeps it deterministic)
+__device__ __forceinline__ ScoreId better_strict(ScoreId a, ScoreId b) { return (b.v > a.v) ? b : a; }
+
+template <int K, int N>
+struct ColTree {  // maximum of N consecutive cells starting at cell K0 (pairwise tree)
+    template <int K0>
+    __device__ __forceinline__ static ScoreId reduce(const double* sc, int lid0) {
+        if constexpr (N == 1) {
+            return ScoreId{sc[K0], lid0 + K0};
+        } else {
+            const ScoreId a = ColTree<K, N / 2>::template reduce<K0>(sc, lid0);
+            const ScoreId b = ColTree<K, N - N / 2>::template reduce<K0 + N / 2>(sc, lid0);
+            return better_strict(a, b);
+        }
+    }
+};
+
+// WRITEBACK: store each diagonal's final QT (the block drop-in's tmpQ); the
+// self-join does not need it and skips the per-cell end-of-diagonal test.
+template <int K, bool WRITEBACK>
 __global__ __launch_bounds__(kGroup) void k_walk_f64(
     const double* __restrict__ Ah, const double* __restrict__ sh, const double* __restrict__ mh,
     const double* __restrict__ Av, const double* __restrict__ sv, const double* __restrict__ mv,
     double* qt, unsigned long long* rowval, unsigned long long* rowkey,
     unsigned long long* colval, unsigned long long* colkey, unsigned int* counter,
-    int64_t row_off, int64_t col_off, int blocklen, int tl, int m, int ib, int nbands,
-    int writeback) {
+    int64_t row_off, int64_t col_off, int blocklen, int tl, int m, int ib, int nbands) {
     constexpr int F = kFlush;
     static_assert(F % K == 0, "flush period must be a multiple of K");
     __shared__ double s_cv[kWavesPerGroup][F][kWave + 1];
     __shared__ int s_ci[kWavesPerGroup][F][kWave + 1];
-    __shared__ double s_rv[kWavesPerGroup][F];
-    __shared__ int s_ri[kWavesPerGroup][F];
+    __shared__ double s_rv[kWavesPerGroup][F + kWave];  // [F, F+64): per-lane dummy slots
+    __shared__ int s_ri[kWavesPerGroup][F + kWave];
 
     const int wv = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     const double NEG = -__builtin_inf();
+    const double QNAN = __builtin_nan("");
     const int nitems = (nbands + 1) / 2;
 
     // Flush the results of columns [jstart, jstart+cnt) and of the rows that
@@ -178,34 +174,25 @@ __global__ __launch_bounds__(kGroup) void k_walk_f64(
     auto flush = [&](int jstart, int cnt, int dbase) {
         __builtin_amdgcn_wave_barrier();
         const int cs = lane & (F - 1), qq = lane / F;
-        double bv = NEG;
-        int bi = -1;
+        ScoreId b{NEG, -1};
         if (cs < cnt) {
 #pragma unroll 2
             for (int u = 0; u < kWave / 4; ++u) {
-                const double v = s_cv[wv][cs][qq * (kWave / 4) + u];
-                const int id = s_ci[wv][cs][qq * (kWave / 4) + u];
-                if (v > bv) {
-                    bv = v;
-                    bi = id;
-                }
+                const ScoreId c{s_cv[wv][cs][qq * (kWave / 4) + u], s_ci[wv][cs][qq * (kWave / 4) + u]};
+                b = better(b, c);
             }
         }
 #pragma unroll
         for (int off = F; off < kWave; off <<= 1) {
-            const double ov = __shfl_xor(bv, off);
-            const int oi = __shfl_xor(bi, off);
-            if (ov > bv || (ov == bv && oi > bi)) {
-                bv = ov;
-                bi = oi;
-            }
+            const ScoreId o{__shfl_xor(b.v, off), __shfl_xor(b.id, off)};
+            b = better(b, o);
         }
-        if (lane < cnt && bi >= 0) {
+        if (lane < cnt && b.id >= 0) {
             const int jj = jstart + lane;
-            const uint64_t o = ord_f64(bv);
+            const uint64_t o = ord_f64(b.v);
             atomicMax(&colval[jj], static_cast<unsigned long long>(o));
             atomicMax(&colkey[jj], static_cast<unsigned long long>(
-                                       make_key(o, ib, static_cast<int64_t>(jj) + dbase + bi + col_off)));
+                                       make_key(o, ib, static_cast<int64_t>(jj) + dbase + b.id + col_off)));
         }
         if (lane < cnt) {
             const double v = s_rv[wv][lane];
@@ -233,80 +220,88 @@ __global__ __launch_bounds__(kGroup) void k_walk_f64(
             const int dbase = band * kWave * K;
             const int L = tl - dbase;
             const int lid0 = lane * K;
-            const int dtop = dbase + kWave * K - 1;
-            const int lim_top = (dtop < blocklen) ? (tl - 1 - dtop) : -1;
-
-            int jlim[K];
+            // Cells outside the triangle are masked by NaN: a row past the last
+            // one gets sInv = NaN, a diagonal past the block gets QT = NaN, and a
+            // NaN score never wins a comparison (nor does it in the reference).
             double q[K], wa[K], wam[K], ws[K], wm[K], rv[K];
             int ri[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int d = dbase + lid0 + k;
-                const bool live = d < blocklen;
-                jlim[k] = live ? tl - 1 - d : -1;
-                q[k] = live ? qt[d] : 0.0;
+                q[k] = (d < blocklen) ? qt[d] : QNAN;
                 const int r = min(d, tl - 1);
                 wa[k] = Av[r];
                 wam[k] = Av[r + m];
-                ws[k] = sv[r];
+                ws[k] = (d < tl) ? sv[r] : QNAN;
                 wm[k] = mv[r];
                 rv[k] = NEG;
                 ri[k] = -1;
             }
 
-            auto macro = [&](int c0, auto full_tag) {
-                constexpr bool FULL = decltype(full_tag)::value;
+            // One step: column j for all K diagonals of every lane.  TAIL steps may
+            // run past the band's last column and stop there.
+            auto step = [&](int j, auto s_tag, auto tail_tag) -> bool {
+                constexpr int S = decltype(s_tag)::value;
+                constexpr bool TAIL = decltype(tail_tag)::value;
+                if (TAIL && j >= L) return false;
+                const double ahj = Ah[j], ahjm = Ah[j + m], shj = sh[j], mhj = mh[j];
+                const int inext = j + dbase + kWave * K;  // row entering lane 63
+                const int ifeed = min(inext, tl - 1);
+                const double fa = Av[ifeed], fam = Av[ifeed + m], fsl = sv[ifeed], fm = mv[ifeed];
+                // ifeed is clamped, so the load is always in bounds; select afterwards
+                const double fs = (inext < tl) ? fsl : QNAN;
+                double sc[K];
 #pragma unroll
-                for (int s = 0; s < K; ++s) {
-                    const int j = c0 + s;
-                    if (!FULL && j >= L) break;
-                    const double ahj = Ah[j], ahjm = Ah[j + m], shj = sh[j], mhj = mh[j];
-                    const int ifeed = min(j + dbase + kWave * K, tl - 1);
-                    const double fa = Av[ifeed], fam = Av[ifeed + m], fs = sv[ifeed], fm = mv[ifeed];
-                    double cv = NEG;
-                    int ci = -1;
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const int sl = (s + k) % K;
-                        const double sc = fma(q[k], ws[sl] * shj, -(wm[sl] * mhj));
-                        q[k] = fma(wam[sl], ahjm, q[k]);
-                        q[k] = fma(-wa[sl], ahj, q[k]);
-                        bool ok = true;
-                        if constexpr (!FULL) {
-                            ok = j <= jlim[k];
-                            if (writeback && j == jlim[k]) qt[dbase + lid0 + k] = q[k];
-                        }
-                        if (ok && sc > rv[sl]) {
-                            rv[sl] = sc;
-                            ri[sl] = lid0 + k;
-                        }
-                        if (ok && sc > cv) {
-                            cv = sc;
-                            ci = lid0 + k;
-                        }
+                for (int k = 0; k < K; ++k) {
+                    const int sl = (S + k) % K;
+                    sc[k] = fma(q[k], ws[sl] * shj, -(wm[sl] * mhj));
+                    q[k] = fma(wam[sl], ahjm, q[k]);
+                    q[k] = fma(-wa[sl], ahj, q[k]);
+                    if constexpr (WRITEBACK) {
+                        if (j == tl - 1 - (dbase + lid0 + k) && dbase + lid0 + k < blocklen)
+                            qt[dbase + lid0 + k] = q[k];
                     }
-                    const int fslot = j & (F - 1);
-                    s_cv[wv][fslot][lane] = cv;
-                    s_ci[wv][fslot][lane] = ci;
-                    if (lane == 0) {
-                        s_rv[wv][fslot] = rv[s];
-                        s_ri[wv][fslot] = ri[s];
+                    if (sc[k] > rv[sl]) {  // NaN never wins
+                        rv[sl] = sc[k];
+                        ri[sl] = lid0 + k;
                     }
-                    rv[s] = shift_down_f64(rv[s], NEG);
-                    ri[s] = shift_down_i32(ri[s], -1);
-                    wa[s] = shift_down_f64(wa[s], fa);
-                    wam[s] = shift_down_f64(wam[s], fam);
-                    ws[s] = shift_down_f64(ws[s], fs);
-                    wm[s] = shift_down_f64(wm[s], fm);
-                    if (fslot == F - 1 || (!FULL && j == L - 1)) flush(j - fslot, fslot + 1, dbase);
                 }
+                const ScoreId col = ColTree<K, K>::template reduce<0>(sc, lid0);
+                const int fslot = j & (F - 1);
+                s_cv[wv][fslot][lane] = col.v;
+                s_ci[wv][fslot][lane] = (col.v > NEG) ? col.id : -1;
+                // lane 0's slot S holds the row leaving the band; the other lanes
+                // write their copy to a private dummy slot (no branch)
+                const int rsel = (lane == 0) ? fslot : F + lane;
+                s_rv[wv][rsel] = rv[S];
+                s_ri[wv][rsel] = ri[S];
+                rv[S] = shift_down_f64(rv[S], NEG);
+                ri[S] = shift_down_i32(ri[S], -1);
+                wa[S] = shift_down_f64(wa[S], fa);
+                wam[S] = shift_down_f64(wam[S], fam);
+                ws[S] = shift_down_f64(ws[S], fs);
+                wm[S] = shift_down_f64(wm[S], fm);
+                return true;
             };
-
-            for (int c0 = 0; c0 < L; c0 += K) {
-                if (c0 + K - 1 < lim_top)
-                    macro(c0, std::integral_constant<bool, true>{});
-                else
-                    macro(c0, std::integral_constant<bool, false>{});
+            auto macro = [&](int c0, auto tail_tag) -> bool {
+                bool go = true;
+                [&]<int... S>(std::integer_sequence<int, S...>) {
+                    ((go = go && step(c0 + S, std::integral_constant<int, S>{}, tail_tag)), ...);
+                }(std::make_integer_sequence<int, K>{});
+                return go;
+            };
+            int c0 = 0;
+            // whole flush windows: no control flow inside a window
+            for (; c0 + F <= L; c0 += F) {
+#pragma unroll 1
+                for (int w = 0; w < F; w += K) macro(c0 + w, std::false_type{});
+                flush(c0, F, dbase);
+            }
+            // the last partial window
+            if (c0 < L) {
+                const int start = c0;
+                for (; c0 < L; c0 += K) macro(c0, std::true_type{});
+                flush(start, L - start, dbase);
             }
         }
     }
@@ -393,10 +388,16 @@ int launch_walk_f64(const WalkF64Launch& a, hipStream_t st) {
     const dim3 grid(a.groups), block(kGroup);
 #define MP_WALK_CASE(KK)                                                                         \
     case KK:                                                                                     \
-        hipLaunchKernelGGL(k_walk_f64<KK>, grid, block, 0, st, a.Ah, a.sh, a.mh, a.Av, a.sv,     \
-                           a.mv, a.qt, a.rowval, a.rowkey, a.colval, a.colkey, a.counter,        \
-                           a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib, nbands,            \
-                           a.writeback);                                                         \
+        if (a.writeback)                                                                         \
+            hipLaunchKernelGGL((k_walk_f64<KK, true>), grid, block, 0, st, a.Ah, a.sh, a.mh,     \
+                               a.Av, a.sv, a.mv, a.qt, a.rowval, a.rowkey, a.colval, a.colkey,   \
+                               a.counter, a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib,     \
+                               nbands);                                                          \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_walk_f64<KK, false>), grid, block, 0, st, a.Ah, a.sh, a.mh,    \
+                               a.Av, a.sv, a.mv, a.qt, a.rowval, a.rowkey, a.colval, a.colkey,   \
+                               a.counter, a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib,     \
+                               nbands);                                                          \
         break;
     switch (a.band_k) {
         MP_WALK_CASE(1)

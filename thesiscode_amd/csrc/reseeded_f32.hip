@@ -24,6 +24,7 @@
 // together with the diagonal that produced it.  The partner follows from the
 // diagonal: column j = i - d for a row, row i = j + d for a column.
 #include <type_traits>
+#include <utility>
 
 #include "common.hpp"
 #include "reseeded_f32.hpp"
@@ -85,20 +86,35 @@ __global__ void k_comp_stats(const SeriesDesc* __restrict__ descs,
         const double* a = A + d.off;
         const int m = d.m;
         const double dm = static_cast<double>(m);
-        const double shift = a[i0];
-        double S = 0.0, Sc = 0.0, Q = 0.0, Qc = 0.0;
-        for (int t = 0; t < m; ++t) {
-            const double x = a[i0 + t] - shift;
-            neu_add(S, Sc, x);
-            neu_add_sq(Q, Qc, x);
-        }
+        // The sums restart from a direct sum at every series-local multiple of
+        // kStatChunk and slide from there, so each window's statistics depend on
+        // the window alone, not on how a launch groups series into threads.
+        double shift = 0.0, S = 0.0, Sc = 0.0, Q = 0.0, Qc = 0.0;
+        auto restart = [&](int w) {
+            shift = a[w];
+            S = Sc = Q = Qc = 0.0;
+            for (int t = 0; t < m; ++t) {
+                const double x = a[w + t] - shift;
+                neu_add(S, Sc, x);
+                neu_add_sq(Q, Qc, x);
+            }
+        };
+        auto slide = [&](int w) {  // window w-1 -> window w
+            const double xo = a[w - 1] - shift, xn = a[w - 1 + m] - shift;
+            neu_add(S, Sc, xn);
+            neu_add(S, Sc, -xo);
+            neu_add_sq(Q, Qc, xn);
+            neu_sub_sq(Q, Qc, xo);
+        };
+        const int anchor = i0 - i0 % kStatChunk;
+        restart(anchor);
+        for (int w = anchor + 1; w <= i0; ++w) slide(w);
         for (int i = i0; i < i1; ++i) {
             if (i > i0) {
-                const double xo = a[i - 1] - shift, xn = a[i - 1 + m] - shift;
-                neu_add(S, Sc, xn);
-                neu_add(S, Sc, -xo);
-                neu_add_sq(Q, Qc, xn);
-                neu_sub_sq(Q, Qc, xo);
+                if (i % kStatChunk == 0)
+                    restart(i);
+                else
+                    slide(i);
             }
             const double sm = (S + Sc) / dm;
             double var = fma(-sm, sm, (Q + Qc) / dm);
@@ -115,7 +131,8 @@ __global__ void k_prep_terms(const SeriesDesc* __restrict__ descs,
                              int64_t nentries, const double* __restrict__ A,
                              const double* __restrict__ mu, const double* __restrict__ sigma,
                              float* __restrict__ df, float* __restrict__ dg,
-                             float* __restrict__ inv) {
+                             float* __restrict__ inv, double* __restrict__ df64,
+                             double* __restrict__ dg64) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= nentries) return;
     const int s = find_series(entry_prefix, nseries, e);
@@ -125,12 +142,17 @@ __global__ void k_prep_terms(const SeriesDesc* __restrict__ descs,
     const int64_t o = d.off + i;
     const double sg = sigma[o];
     inv[o] = (sg > 0.0) ? static_cast<float>(1.0 / (sg * sqrt(static_cast<double>(d.m)))) : 0.0f;
+    float f = 0.0f, g = 0.0f;
     if (i + 1 < d.P) {
-        df[o] = static_cast<float>(0.5 * (a[i + d.m] - a[i]));
-        dg[o] = static_cast<float>((a[i + d.m] - mu[o + 1]) + (a[i] - mu[o]));
-    } else {
-        df[o] = 0.0f;
-        dg[o] = 0.0f;
+        f = static_cast<float>(0.5 * (a[i + d.m] - a[i]));
+        g = static_cast<float>((a[i + d.m] - mu[o + 1]) + (a[i] - mu[o]));
+    }
+    df[o] = f;
+    dg[o] = g;
+    if (df64) {
+        // the same FP32-rounded terms, widened: only the accumulation differs
+        df64[o] = static_cast<double>(f);
+        dg64[o] = static_cast<double>(g);
     }
 }
 
@@ -153,11 +175,33 @@ __device__ __forceinline__ unsigned long long global_key(double k, int64_t partn
            static_cast<unsigned long long>(0xFFFFFFFEu - static_cast<uint32_t>(partner));
 }
 
-template <int K>
-__global__ __launch_bounds__(kGroup) void k_tiles_f32(
+// Maximum of two tile keys.  A key's high word is the FP32 bit pattern of a
+// finite corr + bias, so a key is never a double NaN; the plain instruction
+// then needs none of the quieting that a generic fmax inserts per operand.
+__device__ __forceinline__ double key_max(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+template <typename Acc>
+__device__ __forceinline__ Acc shift_down_acc(Acc src, Acc fill);
+template <>
+__device__ __forceinline__ float shift_down_acc<float>(float src, float fill) {
+    return shift_down_f32(src, fill);
+}
+template <>
+__device__ __forceinline__ double shift_down_acc<double>(double src, double fill) {
+    return shift_down_f64(src, fill);
+}
+
+// Acc = float: FP32 recurrence with FP32 accumulation.  Acc = double: the same
+// FP32-rounded terms accumulated in FP64 (an FP64 FMA issues at the FP32 rate).
+template <int K, typename Acc>
+__global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(4))) void k_tiles_f32(
     const SeriesDesc* __restrict__ descs, const int64_t* __restrict__ tile_prefix,
     const int64_t* __restrict__ band_prefix, int nseries, const double* __restrict__ A,
-    const double* __restrict__ mu, const float* __restrict__ df, const float* __restrict__ dg,
+    const double* __restrict__ mu, const Acc* __restrict__ df, const Acc* __restrict__ dg,
     const float* __restrict__ inv, unsigned long long* keys, unsigned int* counter,
     int64_t ntiles, int T) {
     constexpr int F = kFlush;
@@ -165,7 +209,7 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32(
     static_assert(F % K == 0, "flush period must be a multiple of K");
     static_assert(BW * sizeof(double) <= F * (kWave + 1) * sizeof(double), "seed staging fits");
     __shared__ double s_ck[kWavesPerGroup][F][kWave + 1];  // column partials; seed staging
-    __shared__ double s_rk[kWavesPerGroup][F];
+    __shared__ double s_rk[kWavesPerGroup][2 * F + kWave]; // [0, F) row exits; the rest scratch
 
     const int wv = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
@@ -191,10 +235,6 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32(
         const int ce = min(cb + T, L);
         const int64_t roff = d.off + d.base;  // row (vertical) views
         const int64_t coff = d.off;           // column (horizontal) views
-        // all cells of lane l, diagonal k at column j are valid iff j + k <= lim0
-        const int lim0 = tl - 1 - dbase - lid0;
-        const int dtop = dbase + BW - 1;
-        const int lim_top = (dtop < tl) ? (tl - 1 - dtop) : -1;
 
         // ---- FP64 seeds, strided over lanes for coalesced loads ----
         {
@@ -217,18 +257,30 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32(
             for (int k = 0; k < K; ++k) seeds[lane + kWave * k] = acc[k];
             __builtin_amdgcn_wave_barrier();
         }
-        float cov[K], wdf[K], wdg[K], wiv[K];
+        // Row record: recurrence terms and 1/norm.  In the last tile of a band
+        // some cells lie past the triangle; there each row also carries a bias,
+        // 1 for rows of the triangle and -1e30 past its end, so that an invalid
+        // cell's key is negative and never beats a valid one (no per-cell test).
+        // Every other tile adds the constant 1.
+        Acc cov[K], wdf[K], wdg[K];
+        float wiv[K], wbs[K];
         double rk[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            cov[k] = static_cast<float>(seeds[lid0 + k]);
-            const int r = min(cb + dbase + lid0 + k, tl - 1);
+            cov[k] = static_cast<Acc>(seeds[lid0 + k]);
+            const int row = cb + dbase + lid0 + k;
+            const int r = min(row, tl - 1);
             wdf[k] = df[roff + r];
             wdg[k] = dg[roff + r];
             wiv[k] = inv[roff + r];
+            wbs[k] = (row < tl) ? 1.0f : -1e30f;
             rk[k] = 0.0;
         }
         __builtin_amdgcn_wave_barrier();
+        // lane 0 stores the row leaving the band at s_rk[wv][fslot]; lane l > 0
+        // stores into s_rk[wv][F + l + fslot], a scratch slot nobody reads
+        double* const rk_out = &s_rk[wv][(lane == 0) ? 0 : F + lane];
+        double* const ck_out = &s_ck[wv][0][lane];
 
         auto flush = [&](int jstart, int cnt) {
             __builtin_amdgcn_wave_barrier();
@@ -240,7 +292,7 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32(
             }
 #pragma unroll
             for (int off = F; off < kWave; off <<= 1) bk = fmax(bk, __shfl_xor(bk, off));
-            if (lane < cnt && __double2hiint(bk) != 0) {
+            if (lane < cnt && __double2hiint(bk) > 0) {
                 const int jj = jstart + lane;
                 const int lid = __double2loint(bk);
                 const int64_t partner = static_cast<int64_t>(d.base) + jj + dbase + lid;
@@ -248,7 +300,7 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32(
             }
             if (lane < cnt) {
                 const double rkv = s_rk[wv][lane];
-                if (__double2hiint(rkv) != 0) {
+                if (__double2hiint(rkv) > 0) {
                     const int jj = jstart + lane;
                     const int lid = __double2loint(rkv);
                     atomicMax(&keys[roff + jj + dbase], global_key(rkv, jj - lid));
@@ -257,56 +309,81 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32(
             __builtin_amdgcn_wave_barrier();
         };
 
-        auto macro = [&](int c0, auto full_tag) {
-            constexpr bool FULL = decltype(full_tag)::value;
+        // One column j = c0 + W of the band; W is the step's place in the flush
+        // window, so the row slot (W mod K) and the LDS offsets are constants.
+        // BIASED: the tile reaches past the triangle (feed rows are clamped and
+        // carry the bias).  TAIL: the window may end before F steps.
+        auto step = [&](int c0, auto w_tag, auto biased_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
+            constexpr int W = decltype(w_tag)::value;
+            constexpr int S = W % K;
+            constexpr bool BIASED = decltype(biased_tag)::value;
+            constexpr bool TAIL = decltype(tail_tag)::value;
+            const int j = c0 + W;
+            if (TAIL && j >= ce) return false;
+            const Acc dfh = df[coff + j], dgh = dg[coff + j];
+            const float ivh = inv[coff + j];
+            // row entering lane 63.  Unbiased tiles need no clamp: only the last
+            // step's feed can reach row tl (entry P of the series, inside the
+            // padded arrays) and that row is never used.
+            const int inext = j + dbase + BW;
+            const int ifeed = BIASED ? min(inext, tl - 1) : inext;
+            const Acc fdf = df[roff + ifeed], fdg = dg[roff + ifeed];
+            const float fiv = inv[roff + ifeed];
+            double ck;
 #pragma unroll
-            for (int st = 0; st < K; ++st) {
-                const int j = c0 + st;
-                if (!FULL && j >= ce) break;
-                const float dfh = df[coff + j], dgh = dg[coff + j], ivh = inv[coff + j];
-                const int ifeed = min(j + dbase + BW, tl - 1);
-                const float fdf = df[roff + ifeed], fdg = dg[roff + ifeed], fiv = inv[roff + ifeed];
-                double ck = 0.0;
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const int sl = (st + k) % K;
-                    const float c1 = fmaf(cov[k] * wiv[sl], ivh, 1.0f);
-                    double key = __hiloint2double(__float_as_int(c1), lid0 + k);
-                    cov[k] = fmaf(wdf[sl], dgh, cov[k]);
-                    cov[k] = fmaf(dfh, wdg[sl], cov[k]);
-                    if constexpr (!FULL) key = (j + k <= lim0) ? key : 0.0;
-                    rk[sl] = fmax(rk[sl], key);
-                    ck = fmax(ck, key);
-                }
-                const int fslot = (j - cb) & (F - 1);
-                s_ck[wv][fslot][lane] = ck;
-                if (lane == 0) s_rk[wv][fslot] = rk[st];
-                {
-                    const uint64_t rb = static_cast<uint64_t>(__double_as_longlong(rk[st]));
-                    const int lo = shift_down_i32_zero(static_cast<int>(rb));
-                    const int hi = shift_down_i32_zero(static_cast<int>(rb >> 32));
-                    rk[st] = __hiloint2double(hi, lo);
-                }
-                wdf[st] = shift_down_f32(wdf[st], fdf);
-                wdg[st] = shift_down_f32(wdg[st], fdg);
-                wiv[st] = shift_down_f32(wiv[st], fiv);
-                if (fslot == F - 1 || (!FULL && j == ce - 1)) flush(j - fslot, fslot + 1);
+            for (int k = 0; k < K; ++k) {
+                const int sl = (S + k) % K;
+                const float bias = BIASED ? wbs[sl] : 1.0f;
+                const float c1 = fmaf(static_cast<float>(cov[k]) * wiv[sl], ivh, bias);
+                const double key = __hiloint2double(__float_as_int(c1), lid0 + k);
+                cov[k] = fma(wdf[sl], dgh, cov[k]);
+                cov[k] = fma(dfh, wdg[sl], cov[k]);
+                rk[sl] = key_max(rk[sl], key);
+                ck = (k == 0) ? key : key_max(ck, key);
+            }
+            ck_out[W * (kWave + 1)] = ck;
+            rk_out[W] = rk[S];
+            {
+                const uint64_t rb = static_cast<uint64_t>(__double_as_longlong(rk[S]));
+                const int lo = shift_down_i32_zero(static_cast<int>(rb));
+                const int hi = shift_down_i32_zero(static_cast<int>(rb >> 32));
+                rk[S] = __hiloint2double(hi, lo);
+            }
+            wdf[S] = shift_down_acc<Acc>(wdf[S], fdf);
+            wdg[S] = shift_down_acc<Acc>(wdg[S], fdg);
+            wiv[S] = shift_down_f32(wiv[S], fiv);
+            if (BIASED) wbs[S] = shift_down_f32(wbs[S], (inext < tl) ? 1.0f : -1e30f);
+            return true;
+        };
+        auto window = [&](int c0, auto biased_tag, auto tail_tag) __attribute__((always_inline)) {
+            bool go = true;
+            [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
+                ((go = go && step(c0, std::integral_constant<int, W>{}, biased_tag, tail_tag)), ...);
+            }(std::make_integer_sequence<int, F>{});
+        };
+        auto walk = [&](auto biased_tag) __attribute__((always_inline)) {
+            int c0 = cb;
+            for (; c0 + F <= ce; c0 += F) {
+                window(c0, biased_tag, std::false_type{});
+                flush(c0, F);
+            }
+            if (c0 < ce) {
+                window(c0, biased_tag, std::true_type{});
+                flush(c0, ce - c0);
             }
         };
-
-        for (int c0 = cb; c0 < ce; c0 += K) {
-            if (c0 + K - 1 < lim_top && c0 + K < ce)
-                macro(c0, std::integral_constant<bool, true>{});
-            else
-                macro(c0, std::integral_constant<bool, false>{});
-        }
+        // cell (j, lid) lies in the triangle iff j + lid < L
+        if (ce - 1 + BW - 1 < L)
+            walk(std::false_type{});
+        else
+            walk(std::true_type{});
         // rows still inside the band at the tile's end: slot sl holds row
-        // ce + dbase + lid0 + ((sl - ce) mod K)
+        // ce + dbase + lid0 + ((sl - (ce - cb)) mod K)
 #pragma unroll
         for (int sl = 0; sl < K; ++sl) {
-            const int k = ((sl - ce) % K + K) % K;
+            const int k = ((sl - (ce - cb)) % K + K) % K;
             const int i = ce + dbase + lid0 + k;
-            if (i < tl && __double2hiint(rk[sl]) != 0) {
+            if (i < tl && __double2hiint(rk[sl]) > 0) {
                 const int lid = __double2loint(rk[sl]);
                 atomicMax(&keys[roff + i], global_key(rk[sl], i - dbase - lid));
             }
@@ -333,8 +410,10 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
         const double dm = static_cast<double>(d.m);
         double d2;
         int64_t idx;
-        if (low == 0xFFFFFFFFu) {
-            d2 = 2.0 * dm + 2.0;  // nothing beat the initial score -1.0
+        if (low == 0xFFFFFFFFu || !(sigma[d.off + i] > 0.0)) {
+            // nothing beat the initial score -1.0, or a flat window (zero
+            // variance), whose every score is NaN in FP64 and never wins
+            d2 = 2.0 * dm + 2.0;
             idx = index_init;
         } else if (0xFFFFFFFEu - low >= static_cast<uint32_t>(d.P)) {
             d2 = __builtin_nan("");  // a partner outside the profile: reported, never dereferenced
@@ -369,7 +448,8 @@ int launch_f32r_prep(const F32rBatch& b, hipStream_t st) {
                        0, st, b.descs, b.entry_prefix, b.nseries, b.nentries, b.A, b.mu, b.sigma);
     const unsigned g = static_cast<unsigned>((b.nentries + 255) / 256);
     hipLaunchKernelGGL(k_prep_terms, dim3(g), dim3(256), 0, st, b.descs, b.entry_prefix,
-                       b.nseries, b.nentries, b.A, b.mu, b.sigma, b.df, b.dg, b.inv);
+                       b.nseries, b.nentries, b.A, b.mu, b.sigma, b.df, b.dg, b.inv,
+                       b.accum64 ? b.df64 : nullptr, b.accum64 ? b.dg64 : nullptr);
     hipLaunchKernelGGL(k_init_keys, dim3(g), dim3(256), 0, st, b.descs, b.entry_prefix,
                        b.nseries, b.nentries, b.keys);
     return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
@@ -379,9 +459,14 @@ int launch_f32r_tiles(const F32rBatch& b, int groups, hipStream_t st) {
     if (b.ntiles <= 0) return MP_OK;
 #define MP_TILE_CASE(KK)                                                                        \
     case KK:                                                                                    \
-        hipLaunchKernelGGL(k_tiles_f32<KK>, dim3(groups), dim3(kGroup), 0, st, b.descs,         \
-                           b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu, b.df, b.dg,      \
-                           b.inv, b.keys, b.counter, b.ntiles, b.tile_len);                     \
+        if (b.accum64)                                                                          \
+            hipLaunchKernelGGL((k_tiles_f32<KK, double>), dim3(groups), dim3(kGroup), 0, st,    \
+                               b.descs, b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu,     \
+                               b.df64, b.dg64, b.inv, b.keys, b.counter, b.ntiles, b.tile_len); \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_tiles_f32<KK, float>), dim3(groups), dim3(kGroup), 0, st,     \
+                               b.descs, b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu,     \
+                               b.df, b.dg, b.inv, b.keys, b.counter, b.ntiles, b.tile_len);     \
         break;
     switch (b.band_k) {
         MP_TILE_CASE(4)

@@ -33,6 +33,8 @@ struct F32rBatch {
     const double* A;
     double *mu, *sigma;
     float *df, *dg, *inv;
+    double *df64, *dg64;   // FP64 copies of the terms when the covariance accumulates in FP64
+    int accum64;
     unsigned long long* keys;
     unsigned int* counter;
 };

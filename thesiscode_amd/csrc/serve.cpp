@@ -133,13 +133,15 @@ struct mp_server {
     int device = 0;
     int cus = 256;
     hipStream_t cstream = nullptr, fstream = nullptr;
-    int band_k = 8, tile_len = 1024, refine = 1, excl = -1;
+    int band_k = 8, tile_len = 1024, refine = 1, excl = -1, accum64 = 0;
     int64_t index_init = -1;
     int64_t tick_tiles = 1;
+    int64_t bytes_per_elem = 60;
     // device pools (element-indexed)
     char* dmem = nullptr;
     double *A = nullptr, *mu = nullptr, *sigma = nullptr, *prof = nullptr;
     float *df = nullptr, *dg = nullptr, *inv = nullptr;
+    double *df64 = nullptr, *dg64 = nullptr;
     unsigned long long* keys = nullptr;
     int64_t* idx = nullptr;
     int64_t* bandpool = nullptr;
@@ -350,6 +352,9 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
         pb.df = s->df;
         pb.dg = s->dg;
         pb.inv = s->inv;
+        pb.df64 = s->df64;
+        pb.dg64 = s->dg64;
+        pb.accum64 = s->accum64;
         pb.keys = s->keys;
         if (rc == MP_OK && pb.nentries > 0) rc = mp::launch_f32r_prep(pb, cs);
     }
@@ -370,6 +375,9 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
         tb.df = s->df;
         tb.dg = s->dg;
         tb.inv = s->inv;
+        tb.df64 = s->df64;
+        tb.dg64 = s->dg64;
+        tb.accum64 = s->accum64;
         tb.keys = s->keys;
         tb.counter = s->counter;
         const int groups = static_cast<int>(std::min<int64_t>((ntiles + 3) / 4, 8 * s->cus));
@@ -486,17 +494,19 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     s->tile_len = (o && o->tile_len > 0) ? o->tile_len : 1024;
     if ((s->band_k != 4 && s->band_k != 8 && s->band_k != 16) || s->tile_len % 16 != 0) return fail(MP_ERR_BAD_ARG);
     s->refine = o ? (o->refine != 0) : 1;
+    s->accum64 = o ? (o->accum64 != 0) : 0;
     s->index_init = (o && o->index_init == 0) ? 0 : -1;
     s->excl = o ? o->exclusion : -1;
     const double tile_cells = 64.0 * s->band_k * s->tile_len;
     const double tc = tick_cells > 0 ? static_cast<double>(tick_cells) : 4e9;
     s->tick_tiles = std::max<int64_t>(1, static_cast<int64_t>(tc / tile_cells));
     // device pools: 60 bytes per point
-    const int64_t per_elem = 8 * 3 + 4 * 3 + 8 + 8 + 8;
+    const int64_t per_elem = 8 * 3 + 4 * 3 + 8 + 8 + 8 + (s->accum64 ? 16 : 0);
+    s->bytes_per_elem = per_elem;
     const int64_t cap = static_cast<int64_t>(hbm_budget_bytes / per_elem);
     if (cap < 1024) return fail(MP_ERR_BAD_ARG);
     const int64_t band_cap = cap / 64 + 1024;
-    const size_t bytes = mp::aligned_bytes(cap, 8) * 6 + mp::aligned_bytes(cap, 4) * 3 +
+    const size_t bytes = mp::aligned_bytes(cap, 8) * (s->accum64 ? 8 : 6) + mp::aligned_bytes(cap, 4) * 3 +
                          mp::aligned_bytes(band_cap, 8) + 256;
     if (hipMalloc(&s->dmem, bytes) != hipSuccess) return fail(MP_ERR_OOM);
     char* p = s->dmem;
@@ -514,6 +524,10 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     s->df = reinterpret_cast<float*>(take(mp::aligned_bytes(cap, 4)));
     s->dg = reinterpret_cast<float*>(take(mp::aligned_bytes(cap, 4)));
     s->inv = reinterpret_cast<float*>(take(mp::aligned_bytes(cap, 4)));
+    if (s->accum64) {
+        s->df64 = reinterpret_cast<double*>(take(mp::aligned_bytes(cap, 8)));
+        s->dg64 = reinterpret_cast<double*>(take(mp::aligned_bytes(cap, 8)));
+    }
     s->bandpool = reinterpret_cast<int64_t*>(take(mp::aligned_bytes(band_cap, 8)));
     s->counter = reinterpret_cast<unsigned int*>(take(256));
     s->pool.reset(cap);
@@ -648,7 +662,7 @@ int mp_server_stats(mp_server* s, int64_t* ticks, int64_t* completed, double* ce
     if (cells_done) *cells_done = s->cells_done;
     if (queued) *queued = static_cast<int64_t>(s->waiting.size());
     if (active) *active = static_cast<int64_t>(s->active.size());
-    if (hbm_used_bytes) *hbm_used_bytes = s->pool.used() * 60;
+    if (hbm_used_bytes) *hbm_used_bytes = s->pool.used() * s->bytes_per_elem;
     return s->failed;
 }
 
