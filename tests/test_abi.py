@@ -72,7 +72,7 @@ def test_argument_errors_are_status_codes(name):
 def test_opts_defaults():
     o = Opts()
     lib().mp_opts_default(C.byref(o))
-    assert (o.exclusion, o.refine, o.index_init, o.band_k, o.tile_len) == (-1, 1, -1, 0, 0)
+    assert (o.exclusion, o.refine, o.index_init, o.band_k, o.tile_len, o.accum64) == (-1, 1, -1, 0, 0, 1)
 
 
 def test_philox_known_answers():

@@ -59,12 +59,13 @@ def test_reseed_interval_variants(tile_len, accum64):
     assert erru.max() <= 1e-2
 
 
+@pytest.mark.parametrize("accum64", [False, True])
 @pytest.mark.parametrize("band_k", [4, 8, 16])
-def test_band_width_variants_agree(band_k):
+def test_band_width_variants_agree(band_k, accum64):
     A = tc.random_walk(71, 30000)
     m = 64
-    p_ref, i_ref, _ = tc.selfjoin_f32r(A, m, band_k=8)
-    p, i, _ = tc.selfjoin_f32r(A, m, band_k=band_k)
+    p_ref, i_ref, _ = tc.selfjoin_f32r(A, m, band_k=8, tile_len=2048, accum64=accum64)
+    p, i, _ = tc.selfjoin_f32r(A, m, band_k=band_k, tile_len=2048, accum64=accum64)
     # tiles are seeded identically whatever the band width, so results match exactly
     assert np.array_equal(i, i_ref)
     assert np.array_equal(p, p_ref)

@@ -12,35 +12,10 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "exact_f64.hpp"
 #include "reseeded_f32.hpp"
 #include "scrimp_hip.h"
 
-namespace mp {
-// launchers defined in exact_f64.hip
-int launch_ref_stats(const double* A, int n, int m, double* cs, double* sq, double* sInv,
-                     double* mus, hipStream_t st);
-int launch_ref_seeds(const double* A, int base, int count, int m, double* qt, hipStream_t st);
-int launch_fill_u64(unsigned long long* p, int64_t count, unsigned long long v, hipStream_t st);
-int launch_fill_u32(unsigned int* p, int count, unsigned int v, hipStream_t st);
-int launch_unpack_f64(const unsigned long long* val, const unsigned long long* key, int P, int m,
-                      int ib, int64_t index_init, double* profile, int64_t* index,
-                      hipStream_t st);
-int launch_pack_block(const double* prof, int count, int ib, unsigned long long* val,
-                      unsigned long long* key, hipStream_t st);
-int launch_unpack_block(const unsigned long long* val, const unsigned long long* key, int count,
-                        int ib, int64_t pmin, const int64_t* idx_in, double* prof_out,
-                        int64_t* idx_out, hipStream_t st);
-struct WalkF64Launch {
-    const double *Ah, *sh, *mh, *Av, *sv, *mv;
-    double* qt;
-    unsigned long long *rowval, *rowkey, *colval, *colkey;
-    unsigned int* counter;
-    int64_t row_off, col_off;
-    int blocklen, tl, m, ib, writeback, band_k, groups;
-};
-int walk_f64_nbands(int blocklen, int band_k);
-int launch_walk_f64(const WalkF64Launch& a, hipStream_t st);
-}  // namespace mp
 
 #define MP_TRY(expr)                       \
     do {                                   \
@@ -126,12 +101,27 @@ int geometry(int64_t n, int32_t m, const mp_opts* o, Geometry* g) {
     return MP_OK;
 }
 
+// Band width of the exact walker.  A work item is a pair of bands of 64*K
+// diagonals, so all items hold the same W*64K cells and a SIMD with w items
+// takes w times as long: the time goes as ceil(items / SIMDs) * K * c_K, with
+// c_K the measured cost per cell of width K (relative, MI355X: the row shift
+// is spread over K cells).  Pick the K that minimises it.
 int pick_band_k_f64(const mp_ctx* c, int W, int requested) {
     if (requested > 0) return requested;
-    int k = 4;
-    const int target = 2 * 4 * c->cus;  // two waves per SIMD
-    while (k > 1 && (mp::walk_f64_nbands(W, k) + 1) / 2 < target) k >>= 1;
-    return k;
+    static const int ks[4] = {1, 2, 4, 8};
+    static const double cost[4] = {150.0, 73.0, 44.0, 41.0};
+    const int64_t simds = 4LL * std::max(1, c->cus);
+    int best = 4;
+    double best_t = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        const int64_t items = (mp::walk_f64_nbands(W, ks[i]) + 1) / 2;
+        const double t = static_cast<double>((items + simds - 1) / simds) * ks[i] * cost[i];
+        if (i == 0 || t < best_t) {
+            best_t = t;
+            best = ks[i];
+        }
+    }
+    return best;
 }
 
 // ord(-1.0) in the order-preserving uint64 map used by the exact path's keys
@@ -169,7 +159,7 @@ void mp_opts_default(mp_opts* o) {
     o->exclusion = -1;
     o->refine = 1;
     o->index_init = -1;
-    o->accum64 = 0;
+    o->accum64 = 1;
 }
 
 int mp_device_count(int* count) {
@@ -257,6 +247,7 @@ static int selfjoin_f64_locked(mp_ctx* c, const double* d_ts, const Geometry& g,
     if (band_k != 1 && band_k != 2 && band_k != 4 && band_k != 8) return MP_ERR_BAD_ARG;
     const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n, 8) +
                         2 * aligned_bytes(P + 64, 8) + aligned_bytes(W + 1, 8) +
+                        aligned_bytes(P + mp::kWalkRecPad, sizeof(mp::WalkRec)) +
                         2 * aligned_bytes(P, 8) + aligned_bytes(1, 4);
     MP_TRY(c->work.reserve(need));
     c->work.used = 0;
@@ -266,6 +257,7 @@ static int selfjoin_f64_locked(mp_ctx* c, const double* d_ts, const Geometry& g,
     double* sInv = c->work.take<double>(P + 64);
     double* mus = c->work.take<double>(P + 64);
     double* qt = c->work.take<double>(W + 1);
+    auto* rec = c->work.take<mp::WalkRec>(P + mp::kWalkRecPad);
     auto* val = c->work.take<unsigned long long>(P);
     auto* key = c->work.take<unsigned long long>(P);
     auto* counter = c->work.take<unsigned int>(1);
@@ -275,6 +267,8 @@ static int selfjoin_f64_locked(mp_ctx* c, const double* d_ts, const Geometry& g,
     MP_HIP(hipMemcpyAsync(A, d_ts, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     MP_TRY(mp::launch_fill_u64(reinterpret_cast<unsigned long long*>(A + n), 64, 0ull, st));
     MP_TRY(mp::launch_ref_stats(A, static_cast<int>(n), m, cs, sq, sInv, mus, st));
+    MP_TRY(mp::launch_pack_rec(A, sInv, mus, m, static_cast<int>(P), static_cast<int>(P) + mp::kWalkRecPad,
+                               rec, st));
     MP_HIP(hipEventRecord(c->ev[1], st));
     MP_TRY(mp::launch_ref_seeds(A, g.base, W, m, qt, st));
     const unsigned long long v0 = init_val_f64();
@@ -284,12 +278,8 @@ static int selfjoin_f64_locked(mp_ctx* c, const double* d_ts, const Geometry& g,
     MP_HIP(hipEventRecord(c->ev[2], st));
     if (W > 0) {
         mp::WalkF64Launch a{};
-        a.Ah = A;
-        a.sh = sInv;
-        a.mh = mus;
-        a.Av = A + g.base;
-        a.sv = sInv + g.base;
-        a.mv = mus + g.base;
+        a.colrec = rec;           // column j: window j
+        a.rowrec = rec + g.base;  // row r: window base + r
         a.qt = qt;
         a.rowval = val + g.base;
         a.rowkey = key + g.base;
@@ -379,12 +369,19 @@ int mp_selfjoin_f64(const double* ts, int64_t n, int32_t m, double* profile, int
 // ---------------------------------------------------------------------------
 // FP32-reseeded self-join (a batch of one series)
 // ---------------------------------------------------------------------------
-static int default_tile_len(int m, int requested) {
+// Columns per tile (the re-seed interval): the longest of 8192..512 that still
+// gives every SIMD of the device 4 tiles' worth of waves.  Longer tiles amortise
+// the FP64 seed (m FMAs per diagonal) over more cells.
+static int default_tile_len(int64_t W, int band_k, int cus, int requested) {
     if (requested > 0) return requested;
-    int t = 4 * m;
-    if (t < 1024) t = 1024;
-    if (t > 8192) t = 8192;
-    return (t + 15) & ~15;
+    const int64_t bw = 64LL * band_k;
+    const int64_t target = 16LL * cus;
+    for (int T = 8192; T >= 512; T >>= 1) {
+        int64_t nt = 0;
+        for (int64_t d = 0; d < W; d += bw) nt += (W - d + T - 1) / T;
+        if (nt >= target) return T;
+    }
+    return 256;
 }
 
 int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double* d_profile,
@@ -394,10 +391,10 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     MP_TRY(geometry(n, m, o, &g));
     const int band_k = (o && o->band_k > 0) ? o->band_k : 8;
     if (band_k != 4 && band_k != 8 && band_k != 16) return MP_ERR_BAD_ARG;
-    const int T = default_tile_len(m, o ? o->tile_len : 0);
+    const int T = default_tile_len(g.W, band_k, c->cus, o ? o->tile_len : 0);
     if (T % 16 != 0 || T <= 0) return MP_ERR_BAD_ARG;
     const int refine = o ? (o->refine != 0) : 1;
-    const int accum64 = o ? (o->accum64 != 0) : 0;
+    const int accum64 = o ? (o->accum64 != 0) : 1;
     const int64_t index_init = (o && o->index_init == 0) ? 0 : -1;
     const int BW = 64 * band_k;
     const int nbands = (g.W + BW - 1) / BW;
@@ -549,6 +546,8 @@ int mp_eval_diag_block_triangle_f64(double prof_colmin[], int64_t idx_colmin[],
     const int64_t na = tl + w;  // A arrays: indices 0 .. tl-1+w
     const int64_t nseg = overlap ? 1 : 2;
     const size_t need = 2 * aligned_bytes(na + 1, 8) + 4 * aligned_bytes(tl, 8) +
+                        aligned_bytes(tl + 1, sizeof(mp::WalkRec)) +
+                        aligned_bytes(tl + mp::kWalkRecPad, sizeof(mp::WalkRec)) +
                         aligned_bytes(bl, 8) +
                         nseg * (2 * aligned_bytes(span, 8) + 3 * aligned_bytes(span, 8)) +
                         aligned_bytes(1, 4);
@@ -561,6 +560,8 @@ int mp_eval_diag_block_triangle_f64(double prof_colmin[], int64_t idx_colmin[],
     double* dsv = c->work.take<double>(tl);
     double* dmv = c->work.take<double>(tl);
     double* dq = c->work.take<double>(bl);
+    auto* colrec = c->work.take<mp::WalkRec>(tl + 1);  // +1: the walker reads one column ahead
+    auto* rowrec = c->work.take<mp::WalkRec>(tl + mp::kWalkRecPad);
     unsigned long long* val[2];
     unsigned long long* key[2];
     double* prof[2];
@@ -592,13 +593,11 @@ int mp_eval_diag_block_triangle_f64(double prof_colmin[], int64_t idx_colmin[],
         MP_TRY(mp::launch_pack_block(prof[s], static_cast<int>(span), ib, val[s], key[s], st));
     }
     MP_TRY(mp::launch_fill_u32(counter, 1, 0u, st));
+    MP_TRY(mp::launch_pack_rec(dAh, dsh, dmh, static_cast<int>(w), tl, tl + 1, colrec, st));
+    MP_TRY(mp::launch_pack_rec(dAv, dsv, dmv, static_cast<int>(w), tl, tl + mp::kWalkRecPad, rowrec, st));
     mp::WalkF64Launch a{};
-    a.Ah = dAh;
-    a.sh = dsh;
-    a.mh = dmh;
-    a.Av = dAv;
-    a.sv = dsv;
-    a.mv = dmv;
+    a.colrec = colrec;
+    a.rowrec = rowrec;
     a.qt = dq;
     if (overlap) {
         a.colval = val[0] - lo_off;

@@ -109,7 +109,7 @@ struct mp_batch {
     hipEvent_t ev[3] = {};
     mp::BatchPlan plan;
     int refine = 1;
-    int accum64 = 0;
+    int accum64 = 1;
     int64_t index_init = -1;
     int excl = -1;
     std::vector<int64_t> lengths;
@@ -290,7 +290,7 @@ int mp_batch_create(mp_ctx* ctx, int32_t nseries, const int64_t* lengths, const 
     b->lengths.assign(lengths, lengths + nseries);
     b->windows.assign(windows, windows + nseries);
     b->refine = o ? (o->refine != 0) : 1;
-    b->accum64 = o ? (o->accum64 != 0) : 0;
+    b->accum64 = o ? (o->accum64 != 0) : 1;
     b->index_init = (o && o->index_init == 0) ? 0 : -1;
     b->excl = o ? o->exclusion : -1;
     if (o && o->exclusion < -1) return fail(MP_ERR_BAD_WINDOW);

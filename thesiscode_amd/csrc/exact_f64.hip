@@ -25,27 +25,67 @@
 #include <utility>
 
 #include "common.hpp"
+#include "exact_f64.hpp"
 
 namespace mp {
 
-// One lane scans the whole series in the reference's sequential order (the
-// dependent FP64 add chain is the critical path; the loads are uniform and go
-// through the scalar cache, the stores are single-lane vector stores).
+// One lane scans the whole series in the reference's sequential order: the
+// dependent FP64 add chain is the critical path.  The wave stages the series
+// through LDS in chunks with coalesced vector loads issued one chunk ahead
+// (vector loads retire in order, so the scan never waits on the chunk in
+// flight); lane 0 reads the staged chunk and writes both prefix arrays.
+constexpr int kScanChunk = 2048;
+
 __global__ __launch_bounds__(64) void k_ref_cumsum(const double* __restrict__ A, int n,
                                                   double* __restrict__ cs,
                                                   double* __restrict__ sq) {
-    if (threadIdx.x != 0 || n <= 0) return;
-    double c = A[0];
-    double s = A[0] * A[0];
-    cs[0] = c;
-    sq[0] = s;
+    __shared__ double buf[2][kScanChunk];
+    constexpr int R = kScanChunk / 64;
+    const int lane = threadIdx.x;
+    if (n <= 0) return;
+    const int nchunks = (n + kScanChunk - 1) / kScanChunk;
+    double reg[R];
+    auto load = [&](int ch) {
+        const int base = ch * kScanChunk;
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const int i = base + t * 64 + lane;
+            reg[t] = A[min(i, n - 1)];
+        }
+    };
+    auto stage = [&](int slot) {
+#pragma unroll
+        for (int t = 0; t < R; ++t) buf[slot][t * 64 + lane] = reg[t];
+    };
+    load(0);
+    stage(0);
+    __syncthreads();
+    double c = 0.0, s = 0.0;
+    for (int ch = 0; ch < nchunks; ++ch) {
+        if (ch + 1 < nchunks) load(ch + 1);
+        if (lane == 0) {
+            const int base = ch * kScanChunk;
+            const int cnt = min(kScanChunk, n - base);
+            const double* b = buf[ch & 1];
+            int t = 0;
+            if (ch == 0) {
+                c = b[0];
+                s = b[0] * b[0];
+                cs[0] = c;
+                sq[0] = s;
+                t = 1;
+            }
 #pragma unroll 16
-    for (int i = 1; i < n; ++i) {
-        const double a = A[i];
-        c = a + c;
-        s = fma(a, a, s);
-        cs[i] = c;
-        sq[i] = s;
+            for (; t < cnt; ++t) {
+                const double a = b[t];
+                c = a + c;
+                s = fma(a, a, s);
+                cs[base + t] = c;
+                sq[base + t] = s;
+            }
+        }
+        if (ch + 1 < nchunks) stage((ch + 1) & 1);
+        __syncthreads();
     }
 }
 
@@ -120,6 +160,26 @@ __global__ void k_pack_block(const double* __restrict__ prof, int count, int ib,
     key[i] = o | ((1ull << ib) - 1ull);
 }
 
+__global__ void k_pack_rec(const double* __restrict__ A, const double* __restrict__ s,
+                           const double* __restrict__ mu, int m, int valid, int total,
+                           WalkRec* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    WalkRec r;
+    if (i < valid) {
+        r.a = A[i];
+        r.am = A[i + m];
+        r.s = s[i];
+        r.mu = mu[i];
+    } else {
+        r.a = 0.0;
+        r.am = 0.0;
+        r.s = __builtin_nan("");
+        r.mu = 0.0;
+    }
+    out[i] = r;
+}
+
 // (value, local diagonal id) merge: the larger score wins; exact ties keep the
 // larger id so that merging is symmetric and deterministic.  NaN never wins.
 struct ScoreId {
@@ -130,8 +190,21 @@ __device__ __forceinline__ ScoreId better(ScoreId a, ScoreId b) {
     const bool take = b.v > a.v || (b.v == a.v && b.id > a.id);
     return take ? b : a;
 }
-// strict version for the per-step tree (fixed operand order keeps it deterministic)
-__device__ __forceinline__ ScoreId better_strict(ScoreId a, ScoreId b) { return (b.v > a.v) ? b : a; }
+// Maximum of two scores with the plain instruction: it returns the other
+// operand when one is a quiet NaN (a masked cell), and the scores here carry
+// no signalling NaNs, so the per-operand quieting a generic fmax adds is not
+// needed.
+__device__ __forceinline__ double dmax(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// Per-step merge of cells in diagonal order: the larger score wins, the left
+// (smaller diagonal) operand on ties, and a NaN loses to any number.
+__device__ __forceinline__ ScoreId merge_left(ScoreId a, ScoreId b) {
+    const double v = dmax(a.v, b.v);
+    return ScoreId{v, (a.v == v) ? a.id : b.id};
+}
 
 template <int K, int N>
 struct ColTree {  // maximum of N consecutive cells starting at cell K0 (pairwise tree)
@@ -142,7 +215,7 @@ struct ColTree {  // maximum of N consecutive cells starting at cell K0 (pairwis
         } else {
             const ScoreId a = ColTree<K, N / 2>::template reduce<K0>(sc, lid0);
             const ScoreId b = ColTree<K, N - N / 2>::template reduce<K0 + N / 2>(sc, lid0);
-            return better_strict(a, b);
+            return merge_left(a, b);
         }
     }
 };
@@ -150,35 +223,56 @@ struct ColTree {  // maximum of N consecutive cells starting at cell K0 (pairwis
 // WRITEBACK: store each diagonal's final QT (the block drop-in's tmpQ); the
 // self-join does not need it and skips the per-cell end-of-diagonal test.
 template <int K, bool WRITEBACK>
-__global__ __launch_bounds__(kGroup) void k_walk_f64(
-    const double* __restrict__ Ah, const double* __restrict__ sh, const double* __restrict__ mh,
-    const double* __restrict__ Av, const double* __restrict__ sv, const double* __restrict__ mv,
+__device__ __forceinline__ void walk_body(
+    const WalkRec* __restrict__ colrec, const WalkRec* __restrict__ rowrec,
     double* qt, unsigned long long* rowval, unsigned long long* rowkey,
     unsigned long long* colval, unsigned long long* colkey, unsigned int* counter,
     int64_t row_off, int64_t col_off, int blocklen, int tl, int m, int ib, int nbands) {
     constexpr int F = kFlush;
+    constexpr int BW = kWave * K;
     static_assert(F % K == 0, "flush period must be a multiple of K");
-    __shared__ double s_cv[kWavesPerGroup][F][kWave + 1];
-    __shared__ int s_ci[kWavesPerGroup][F][kWave + 1];
-    __shared__ double s_rv[kWavesPerGroup][F + kWave];  // [F, F+64): per-lane dummy slots
-    __shared__ int s_ri[kWavesPerGroup][F + kWave];
+    // Column partials, one row of 64 per step.  Unpadded (53,760 bytes of LDS
+    // per group, so that 3 groups fit in a CU's 160 KiB); the flush reads each
+    // row in a lane-rotated order instead, which spreads the banks.
+    __shared__ double s_cv[kWavesPerGroup][F][kWave];
+    __shared__ int s_ci[kWavesPerGroup][F][kWave];
+    // [0, F): rows leaving lane 0; lane l > 0 writes scratch slot F + l + step
+    __shared__ double s_rv[kWavesPerGroup][2 * F + kWave];
+    __shared__ int s_ri[kWavesPerGroup][2 * F + kWave];
 
     const int wv = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     const double NEG = -__builtin_inf();
     const double QNAN = __builtin_nan("");
     const int nitems = (nbands + 1) / 2;
+    const int lid0 = lane * K;
+    double* const cv_out = &s_cv[wv][0][lane];
+    int* const ci_out = &s_ci[wv][0][lane];
+    double* const rv_out = &s_rv[wv][(lane == 0) ? 0 : F + lane];
+    int* const ri_out = &s_ri[wv][(lane == 0) ? 0 : F + lane];
 
     // Flush the results of columns [jstart, jstart+cnt) and of the rows that
-    // left lane 0 at those columns.
-    auto flush = [&](int jstart, int cnt, int dbase) {
+    // left lane 0 at those columns.  Row ids are stored as id+1 (0 = none).
+    // pc/pr: the current results of the window's columns and rows, read when
+    // the window starts so that the read has landed by the flush.
+    // pc/pr: the current results of this window's columns and rows, read one
+    // window ahead.  Results only grow, so a plain (possibly stale) read filters
+    // out the partials that cannot win; a stale read only lets an unneeded
+    // atomic through.  The test compares the truncated scores the keys hold, so
+    // which keys compete does not depend on timing.  `reload` issues the next
+    // window's reads after the tests and before the atomics: the loads then
+    // reuse pc/pr's registers, and no load has to wait for an atomic.
+    auto flush = [&](int jstart, int cnt, int dbase, uint64_t& pc, uint64_t& pr, auto reload)
+        __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
         const int cs = lane & (F - 1), qq = lane / F;
         ScoreId b{NEG, -1};
         if (cs < cnt) {
+            // better() is a maximum over (score, id), so the order is free
 #pragma unroll 2
             for (int u = 0; u < kWave / 4; ++u) {
-                const ScoreId c{s_cv[wv][cs][qq * (kWave / 4) + u], s_ci[wv][cs][qq * (kWave / 4) + u]};
+                const int e = qq * (kWave / 4) + ((u + cs) & (kWave / 4 - 1));
+                const ScoreId c{s_cv[wv][cs][e], s_ci[wv][cs][e]};
                 b = better(b, c);
             }
         }
@@ -187,24 +281,26 @@ __global__ __launch_bounds__(kGroup) void k_walk_f64(
             const ScoreId o{__shfl_xor(b.v, off), __shfl_xor(b.id, off)};
             b = better(b, o);
         }
-        if (lane < cnt && b.id >= 0) {
-            const int jj = jstart + lane;
-            const uint64_t o = ord_f64(b.v);
-            atomicMax(&colval[jj], static_cast<unsigned long long>(o));
+        const int jj = jstart + lane;
+        const uint64_t oc = ord_f64(b.v);
+        // (non-short-circuit forms: both tests are done before the reload)
+        const bool do_col = (lane < cnt) & (b.id >= 0) & ((oc >> ib) >= (pc >> ib));
+        const double rvl = s_rv[wv][lane & (F - 1)];
+        const int rid = s_ri[wv][lane & (F - 1)] - 1;
+        const uint64_t orw = ord_f64(rvl);
+        const bool do_row = (lane < cnt) & (rid >= 0) & ((orw >> ib) >= (pr >> ib));
+        __builtin_amdgcn_sched_barrier(0);  // tests first: the reload then reuses pc/pr
+        reload(pc, pr);
+        if (do_col) {
+            atomicMax(&colval[jj], static_cast<unsigned long long>(oc));
             atomicMax(&colkey[jj], static_cast<unsigned long long>(
-                                       make_key(o, ib, static_cast<int64_t>(jj) + dbase + b.id + col_off)));
+                                       make_key(oc, ib, static_cast<int64_t>(jj) + dbase + b.id + col_off)));
         }
-        if (lane < cnt) {
-            const double v = s_rv[wv][lane];
-            const int id = s_ri[wv][lane];
-            if (id >= 0) {
-                const int jj = jstart + lane;
-                const int i = jj + dbase;
-                const uint64_t o = ord_f64(v);
-                atomicMax(&rowval[i], static_cast<unsigned long long>(o));
-                atomicMax(&rowkey[i], static_cast<unsigned long long>(
-                                          make_key(o, ib, static_cast<int64_t>(jj - id) + row_off)));
-            }
+        if (do_row) {
+            const int i = jj + dbase;
+            atomicMax(&rowval[i], static_cast<unsigned long long>(orw));
+            atomicMax(&rowkey[i], static_cast<unsigned long long>(
+                                      make_key(orw, ib, static_cast<int64_t>(jj - rid) + row_off)));
         }
         __builtin_amdgcn_wave_barrier();
     };
@@ -217,39 +313,58 @@ __global__ __launch_bounds__(kGroup) void k_walk_f64(
         for (int h = 0; h < 2; ++h) {
             const int band = (h == 0) ? item : nbands - 1 - item;
             if (h == 1 && band == item) break;
-            const int dbase = band * kWave * K;
+            const int dbase = band * BW;
             const int L = tl - dbase;
-            const int lid0 = lane * K;
             // Cells outside the triangle are masked by NaN: a row past the last
-            // one gets sInv = NaN, a diagonal past the block gets QT = NaN, and a
-            // NaN score never wins a comparison (nor does it in the reference).
+            // one is a padding record with sInv = NaN, a diagonal past the block
+            // gets QT = NaN, and a NaN score never wins a comparison (nor does
+            // it in the reference).
             double q[K], wa[K], wam[K], ws[K], wm[K], rv[K];
             int ri[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int d = dbase + lid0 + k;
                 q[k] = (d < blocklen) ? qt[d] : QNAN;
-                const int r = min(d, tl - 1);
-                wa[k] = Av[r];
-                wam[k] = Av[r + m];
-                ws[k] = (d < tl) ? sv[r] : QNAN;
-                wm[k] = mv[r];
+                const WalkRec r = rowrec[d];
+                wa[k] = r.a;
+                wam[k] = r.am;
+                ws[k] = r.s;
+                wm[k] = r.mu;
                 rv[k] = NEG;
-                ri[k] = -1;
+                ri[k] = 0;
             }
 
-            // One step: column j for all K diagonals of every lane.  TAIL steps may
-            // run past the band's last column and stop there.
-            auto step = [&](int j, auto s_tag, auto tail_tag) -> bool {
-                constexpr int S = decltype(s_tag)::value;
+            // One column j = c0 + W for all K diagonals of every lane; W is the
+            // step's place in the flush window, so the row slot (W mod K) and
+            // the LDS offsets are constants.  TAIL: the window may end before
+            // F steps.  Column and feed records are two linear scalar streams.
+            WalkRec cur_c, cur_f;  // the next step's column and feed records
+            auto step = [&](int c0, auto w_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
+                constexpr int W = decltype(w_tag)::value;
+                constexpr int S = W % K;
                 constexpr bool TAIL = decltype(tail_tag)::value;
+                const int j = c0 + W;
                 if (TAIL && j >= L) return false;
-                const double ahj = Ah[j], ahjm = Ah[j + m], shj = sh[j], mhj = mh[j];
-                const int inext = j + dbase + kWave * K;  // row entering lane 63
-                const int ifeed = min(inext, tl - 1);
-                const double fa = Av[ifeed], fam = Av[ifeed + m], fsl = sv[ifeed], fm = mv[ifeed];
-                // ifeed is clamped, so the load is always in bounds; select afterwards
-                const double fs = (inext < tl) ? fsl : QNAN;
+                // records of this column and of the row entering lane 63 (a
+                // padding record past the triangle) were loaded one step ago;
+                // load the next step's now, so the scalar loads overlap a step
+                // Steps are not interleaved by the scheduler: that keeps each
+                // step's use of its records (and so the wait for them) after the
+                // previous step's loads have had a whole step to land.
+                __builtin_amdgcn_sched_barrier(0);
+                const WalkRec c = cur_c, f = cur_f;
+                // Scalar loads may return out of order, so a wait for this step's
+                // records waits for every scalar load in flight: take the wait
+                // here (the empty asm needs the values), before the next step's
+                // loads are issued (its memory clobber keeps them below).
+                asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu), "s"(f.a), "s"(f.am),
+                             "s"(f.s), "s"(f.mu)
+                             : "memory");
+                cur_c = colrec[j + 1];
+                cur_f = rowrec[j + 1 + dbase + BW];
+                __builtin_amdgcn_sched_barrier(0);  // issue them before the step's work
+                const double ahj = c.a, ahjm = c.am, shj = c.s, mhj = c.mu;
+                const double fa = f.a, fam = f.am, fs = f.s, fm = f.mu;
                 double sc[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -261,51 +376,81 @@ __global__ __launch_bounds__(kGroup) void k_walk_f64(
                         if (j == tl - 1 - (dbase + lid0 + k) && dbase + lid0 + k < blocklen)
                             qt[dbase + lid0 + k] = q[k];
                     }
-                    if (sc[k] > rv[sl]) {  // NaN never wins
-                        rv[sl] = sc[k];
-                        ri[sl] = lid0 + k;
-                    }
+                    // the first (smallest-column) maximum wins; NaN never does
+                    ri[sl] = (sc[k] > rv[sl]) ? lid0 + k + 1 : ri[sl];
+                    rv[sl] = dmax(rv[sl], sc[k]);
                 }
                 const ScoreId col = ColTree<K, K>::template reduce<0>(sc, lid0);
-                const int fslot = j & (F - 1);
-                s_cv[wv][fslot][lane] = col.v;
-                s_ci[wv][fslot][lane] = (col.v > NEG) ? col.id : -1;
-                // lane 0's slot S holds the row leaving the band; the other lanes
-                // write their copy to a private dummy slot (no branch)
-                const int rsel = (lane == 0) ? fslot : F + lane;
-                s_rv[wv][rsel] = rv[S];
-                s_ri[wv][rsel] = ri[S];
+                cv_out[W * kWave] = col.v;
+                ci_out[W * kWave] = col.id;
+                rv_out[W] = rv[S];
+                ri_out[W] = ri[S];
                 rv[S] = shift_down_f64(rv[S], NEG);
-                ri[S] = shift_down_i32(ri[S], -1);
+                ri[S] = shift_down_i32_zero(ri[S]);
                 wa[S] = shift_down_f64(wa[S], fa);
                 wam[S] = shift_down_f64(wam[S], fam);
                 ws[S] = shift_down_f64(ws[S], fs);
                 wm[S] = shift_down_f64(wm[S], fm);
                 return true;
             };
-            auto macro = [&](int c0, auto tail_tag) -> bool {
+            auto window = [&](int c0, auto tail_tag) __attribute__((always_inline)) {
                 bool go = true;
-                [&]<int... S>(std::integer_sequence<int, S...>) {
-                    ((go = go && step(c0 + S, std::integral_constant<int, S>{}, tail_tag)), ...);
-                }(std::make_integer_sequence<int, K>{});
-                return go;
+                [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
+                    ((go = go && step(c0, std::integral_constant<int, W>{}, tail_tag)), ...);
+                }(std::make_integer_sequence<int, F>{});
+            };
+            // results of column c0 + (lane mod F) and of its row, clamped in bounds
+            auto current = [&](int c0, uint64_t& pc, uint64_t& pr) __attribute__((always_inline)) {
+                const int jj = min(c0 + (lane & (F - 1)), L - 1);
+                pc = colval[jj];
+                pr = rowval[jj + dbase];
             };
             int c0 = 0;
+            uint64_t pc, pr;
+            current(0, pc, pr);
+            cur_c = colrec[0];
+            cur_f = rowrec[dbase + BW];
+            // let the band's set-up loads land here, so the windows never wait on
+            // vector memory (whose counter would also cover the flush's atomics)
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
             // whole flush windows: no control flow inside a window
             for (; c0 + F <= L; c0 += F) {
-#pragma unroll 1
-                for (int w = 0; w < F; w += K) macro(c0 + w, std::false_type{});
-                flush(c0, F, dbase);
+                window(c0, std::false_type{});
+                flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
+                    current(c0 + F, a, b);
+                });
             }
             // the last partial window
             if (c0 < L) {
-                const int start = c0;
-                for (; c0 < L; c0 += K) macro(c0, std::true_type{});
-                flush(start, L - start, dbase);
+                window(c0, std::true_type{});
+                flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
             }
         }
     }
 }
+
+#define MP_WALK_PARAMS                                                                         \
+    const WalkRec *__restrict__ colrec, const WalkRec *__restrict__ rowrec, double *qt,          \
+        unsigned long long *rowval, unsigned long long *rowkey, unsigned long long *colval,     \
+        unsigned long long *colkey, unsigned int *counter, int64_t row_off, int64_t col_off,    \
+        int blocklen, int tl, int m, int ib, int nbands
+#define MP_WALK_ARGS                                                                           \
+    colrec, rowrec, qt, rowval, rowkey, colval, colkey, counter, row_off, col_off, blocklen, tl, \
+        m, ib, nbands
+
+// K <= 4: at most 168 VGPRs, so that the 3 groups the LDS allows per CU all fit.
+template <int K, bool WRITEBACK>
+__global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_f64(
+    MP_WALK_PARAMS) {
+    walk_body<K, WRITEBACK>(MP_WALK_ARGS);
+}
+// K = 8 holds twice the row records; capping its registers would spill.
+template <int K, bool WRITEBACK>
+__global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(2))) void k_walk_f64_wide(MP_WALK_PARAMS) {
+    walk_body<K, WRITEBACK>(MP_WALK_ARGS);
+}
+#undef MP_WALK_PARAMS
+#undef MP_WALK_ARGS
 
 }  // namespace mp
 
@@ -317,7 +462,7 @@ namespace mp {
 int launch_ref_stats(const double* A, int n, int m, double* cs, double* sq, double* sInv,
                      double* mus, hipStream_t st) {
     const int P = n - m + 1;
-    hipLaunchKernelGGL(k_ref_cumsum, dim3(1), dim3(64), 0, st, A, n, cs, sq);
+    hipLaunchKernelGGL(k_ref_cumsum, dim3(1), dim3(64), 0, st, A, n, cs, sq);  // one wave
     hipLaunchKernelGGL(k_ref_stats, dim3((P + 255) / 256), dim3(256), 0, st, cs, sq, P, m, sInv,
                        mus);
     return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
@@ -368,14 +513,14 @@ int launch_unpack_block(const unsigned long long* val, const unsigned long long*
     return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
 }
 
-struct WalkF64Launch {
-    const double *Ah, *sh, *mh, *Av, *sv, *mv;
-    double* qt;
-    unsigned long long *rowval, *rowkey, *colval, *colkey;
-    unsigned int* counter;
-    int64_t row_off, col_off;
-    int blocklen, tl, m, ib, writeback, band_k, groups;
-};
+
+int launch_pack_rec(const double* A, const double* s, const double* mu, int m, int valid,
+                    int total, WalkRec* out, hipStream_t st) {
+    if (total <= 0) return MP_OK;
+    hipLaunchKernelGGL(k_pack_rec, dim3((total + 255) / 256), dim3(256), 0, st, A, s, mu, m,
+                       valid, total, out);
+    return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
+}
 
 int walk_f64_nbands(int blocklen, int band_k) {
     const int bw = kWave * band_k;
@@ -386,24 +531,22 @@ int launch_walk_f64(const WalkF64Launch& a, hipStream_t st) {
     const int nbands = walk_f64_nbands(a.blocklen, a.band_k);
     if (nbands <= 0) return MP_OK;
     const dim3 grid(a.groups), block(kGroup);
-#define MP_WALK_CASE(KK)                                                                         \
+#define MP_WALK_CASE(KK, KERNEL)                                                               \
     case KK:                                                                                     \
         if (a.writeback)                                                                         \
-            hipLaunchKernelGGL((k_walk_f64<KK, true>), grid, block, 0, st, a.Ah, a.sh, a.mh,     \
-                               a.Av, a.sv, a.mv, a.qt, a.rowval, a.rowkey, a.colval, a.colkey,   \
-                               a.counter, a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib,     \
-                               nbands);                                                          \
+            hipLaunchKernelGGL((KERNEL<KK, true>), grid, block, 0, st, a.colrec, a.rowrec, a.qt, \
+                               a.rowval, a.rowkey, a.colval, a.colkey, a.counter, a.row_off,     \
+                               a.col_off, a.blocklen, a.tl, a.m, a.ib, nbands);                  \
         else                                                                                     \
-            hipLaunchKernelGGL((k_walk_f64<KK, false>), grid, block, 0, st, a.Ah, a.sh, a.mh,    \
-                               a.Av, a.sv, a.mv, a.qt, a.rowval, a.rowkey, a.colval, a.colkey,   \
-                               a.counter, a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib,     \
-                               nbands);                                                          \
+            hipLaunchKernelGGL((KERNEL<KK, false>), grid, block, 0, st, a.colrec, a.rowrec,      \
+                               a.qt, a.rowval, a.rowkey, a.colval, a.colkey, a.counter,          \
+                               a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib, nbands);       \
         break;
     switch (a.band_k) {
-        MP_WALK_CASE(1)
-        MP_WALK_CASE(2)
-        MP_WALK_CASE(4)
-        MP_WALK_CASE(8)
+        MP_WALK_CASE(1, k_walk_f64)
+        MP_WALK_CASE(2, k_walk_f64)
+        MP_WALK_CASE(4, k_walk_f64)
+        MP_WALK_CASE(8, k_walk_f64_wide)
         default:
             return MP_ERR_BAD_ARG;
     }

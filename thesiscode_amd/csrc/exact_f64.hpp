@@ -1,0 +1,47 @@
+// Host-side launchers of the exact FP64 path (defined in exact_f64.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mp {
+
+// One window's record as the band walker reads it: A[i], A[i+m], 1/sigma and
+// mu*sqrt(m)/sigma.  Rows past the triangle are padding records with
+// 1/sigma = NaN, which masks every cell they touch.
+struct alignas(32) WalkRec {
+    double a, am, s, mu;
+};
+// padding records needed past the last row: one band of the widest walker
+constexpr int kWalkRecPad = 64 * 8 + 64;
+
+int launch_ref_stats(const double* A, int n, int m, double* cs, double* sq, double* sInv,
+                     double* mus, hipStream_t st);
+int launch_ref_seeds(const double* A, int base, int count, int m, double* qt, hipStream_t st);
+int launch_fill_u64(unsigned long long* p, int64_t count, unsigned long long v, hipStream_t st);
+int launch_fill_u32(unsigned int* p, int count, unsigned int v, hipStream_t st);
+int launch_unpack_f64(const unsigned long long* val, const unsigned long long* key, int P, int m,
+                      int ib, int64_t index_init, double* profile, int64_t* index,
+                      hipStream_t st);
+int launch_pack_block(const double* prof, int count, int ib, unsigned long long* val,
+                      unsigned long long* key, hipStream_t st);
+int launch_unpack_block(const unsigned long long* val, const unsigned long long* key, int count,
+                        int ib, int64_t pmin, const int64_t* idx_in, double* prof_out,
+                        int64_t* idx_out, hipStream_t st);
+// out[i] = {A[i], A[i+m], s[i], mu[i]} for i < valid, padding records up to total
+int launch_pack_rec(const double* A, const double* s, const double* mu, int m, int valid,
+                    int total, WalkRec* out, hipStream_t st);
+
+struct WalkF64Launch {
+    const WalkRec* colrec;  // column j: colrec[j], j < tl
+    const WalkRec* rowrec;  // row r: rowrec[r], r < tl + kWalkRecPad (padding past tl)
+    double* qt;
+    unsigned long long *rowval, *rowkey, *colval, *colkey;
+    unsigned int* counter;
+    int64_t row_off, col_off;
+    int blocklen, tl, m, ib, writeback, band_k, groups;
+};
+int walk_f64_nbands(int blocklen, int band_k);
+int launch_walk_f64(const WalkF64Launch& a, hipStream_t st);
+
+}  // namespace mp

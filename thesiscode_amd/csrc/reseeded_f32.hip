@@ -198,7 +198,7 @@ __device__ __forceinline__ double shift_down_acc<double>(double src, double fill
 // Acc = float: FP32 recurrence with FP32 accumulation.  Acc = double: the same
 // FP32-rounded terms accumulated in FP64 (an FP64 FMA issues at the FP32 rate).
 template <int K, typename Acc>
-__global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(4))) void k_tiles_f32(
+__device__ __forceinline__ void tiles_body(
     const SeriesDesc* __restrict__ descs, const int64_t* __restrict__ tile_prefix,
     const int64_t* __restrict__ band_prefix, int nseries, const double* __restrict__ A,
     const double* __restrict__ mu, const Acc* __restrict__ df, const Acc* __restrict__ dg,
@@ -282,7 +282,9 @@ __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(4))) voi
         double* const rk_out = &s_rk[wv][(lane == 0) ? 0 : F + lane];
         double* const ck_out = &s_ck[wv][0][lane];
 
-        auto flush = [&](int jstart, int cnt) {
+        // pc/pr: the current keys of the window's columns and rows, read when
+        // the window starts so that the read has landed by the flush
+        auto flush = [&](int jstart, int cnt, unsigned long long pc, unsigned long long pr) {
             __builtin_amdgcn_wave_barrier();
             const int cs = lane & (F - 1), qq = lane / F;
             double bk = 0.0;
@@ -296,14 +298,18 @@ __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const int jj = jstart + lane;
                 const int lid = __double2loint(bk);
                 const int64_t partner = static_cast<int64_t>(d.base) + jj + dbase + lid;
-                atomicMax(&keys[coff + jj], global_key(bk, partner));
+                // keys only grow: a plain (possibly stale) read filters the
+                // partials that cannot win
+                const unsigned long long gk = global_key(bk, partner);
+                if (gk > pc) atomicMax(&keys[coff + jj], gk);
             }
             if (lane < cnt) {
                 const double rkv = s_rk[wv][lane];
                 if (__double2hiint(rkv) > 0) {
                     const int jj = jstart + lane;
                     const int lid = __double2loint(rkv);
-                    atomicMax(&keys[roff + jj + dbase], global_key(rkv, jj - lid));
+                    const unsigned long long gk = global_key(rkv, jj - lid);
+                    if (gk > pr) atomicMax(&keys[roff + jj + dbase], gk);
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -361,15 +367,24 @@ __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(4))) voi
                 ((go = go && step(c0, std::integral_constant<int, W>{}, biased_tag, tail_tag)), ...);
             }(std::make_integer_sequence<int, F>{});
         };
+        auto current = [&](int c0, unsigned long long& pc, unsigned long long& pr) __attribute__((always_inline)) {
+            const int jj = min(c0 + (lane & (F - 1)), ce - 1);
+            pc = keys[coff + jj];
+            pr = keys[roff + jj + dbase];
+        };
         auto walk = [&](auto biased_tag) __attribute__((always_inline)) {
             int c0 = cb;
             for (; c0 + F <= ce; c0 += F) {
+                unsigned long long pc, pr;
+                current(c0, pc, pr);
                 window(c0, biased_tag, std::false_type{});
-                flush(c0, F);
+                flush(c0, F, pc, pr);
             }
             if (c0 < ce) {
+                unsigned long long pc, pr;
+                current(c0, pc, pr);
                 window(c0, biased_tag, std::true_type{});
-                flush(c0, ce - c0);
+                flush(c0, ce - c0, pc, pr);
             }
         };
         // cell (j, lid) lies in the triangle iff j + lid < L
@@ -385,11 +400,34 @@ __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(4))) voi
             const int i = ce + dbase + lid0 + k;
             if (i < tl && __double2hiint(rk[sl]) > 0) {
                 const int lid = __double2loint(rk[sl]);
-                atomicMax(&keys[roff + i], global_key(rk[sl], i - dbase - lid));
+                const unsigned long long gk = global_key(rk[sl], i - dbase - lid);
+                if (gk > keys[roff + i]) atomicMax(&keys[roff + i], gk);
             }
         }
     }
 }
+
+#define MP_TILE_PARAMS                                                                          \
+    const SeriesDesc *__restrict__ descs, const int64_t *__restrict__ tile_prefix,              \
+        const int64_t *__restrict__ band_prefix, int nseries, const double *__restrict__ A,     \
+        const double *__restrict__ mu, const Acc *__restrict__ df, const Acc *__restrict__ dg,  \
+        const float *__restrict__ inv, unsigned long long *keys, unsigned int *counter,         \
+        int64_t ntiles, int T
+#define MP_TILE_ARGS descs, tile_prefix, band_prefix, nseries, A, mu, df, dg, inv, keys, counter, ntiles, T
+
+// K <= 8: at most 128 VGPRs, so that 4 waves share each SIMD.
+template <int K, typename Acc>
+__global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(4))) void k_tiles_f32(
+    MP_TILE_PARAMS) {
+    tiles_body<K, Acc>(MP_TILE_ARGS);
+}
+// K = 16 holds twice the row records; capping its registers would spill.
+template <int K, typename Acc>
+__global__ __launch_bounds__(kGroup) void k_tiles_f32_wide(MP_TILE_PARAMS) {
+    tiles_body<K, Acc>(MP_TILE_ARGS);
+}
+#undef MP_TILE_PARAMS
+#undef MP_TILE_ARGS
 
 // One wave per profile entry.  d^2 = 2m(1 - rho): rho from the FP32 key, or
 // recomputed in FP64 from the centred dot product of the winning pair.
@@ -457,21 +495,21 @@ int launch_f32r_prep(const F32rBatch& b, hipStream_t st) {
 
 int launch_f32r_tiles(const F32rBatch& b, int groups, hipStream_t st) {
     if (b.ntiles <= 0) return MP_OK;
-#define MP_TILE_CASE(KK)                                                                        \
+#define MP_TILE_CASE(KK, KERNEL)                                                                \
     case KK:                                                                                    \
         if (b.accum64)                                                                          \
-            hipLaunchKernelGGL((k_tiles_f32<KK, double>), dim3(groups), dim3(kGroup), 0, st,    \
+            hipLaunchKernelGGL((KERNEL<KK, double>), dim3(groups), dim3(kGroup), 0, st,         \
                                b.descs, b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu,     \
                                b.df64, b.dg64, b.inv, b.keys, b.counter, b.ntiles, b.tile_len); \
         else                                                                                    \
-            hipLaunchKernelGGL((k_tiles_f32<KK, float>), dim3(groups), dim3(kGroup), 0, st,     \
+            hipLaunchKernelGGL((KERNEL<KK, float>), dim3(groups), dim3(kGroup), 0, st,          \
                                b.descs, b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu,     \
                                b.df, b.dg, b.inv, b.keys, b.counter, b.ntiles, b.tile_len);     \
         break;
     switch (b.band_k) {
-        MP_TILE_CASE(4)
-        MP_TILE_CASE(8)
-        MP_TILE_CASE(16)
+        MP_TILE_CASE(4, k_tiles_f32)
+        MP_TILE_CASE(8, k_tiles_f32)
+        MP_TILE_CASE(16, k_tiles_f32_wide)
         default:
             return MP_ERR_BAD_ARG;
     }

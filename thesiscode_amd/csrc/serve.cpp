@@ -133,7 +133,7 @@ struct mp_server {
     int device = 0;
     int cus = 256;
     hipStream_t cstream = nullptr, fstream = nullptr;
-    int band_k = 8, tile_len = 1024, refine = 1, excl = -1, accum64 = 0;
+    int band_k = 8, tile_len = 1024, refine = 1, excl = -1, accum64 = 1;
     int64_t index_init = -1;
     int64_t tick_tiles = 1;
     int64_t bytes_per_elem = 60;
@@ -494,7 +494,7 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     s->tile_len = (o && o->tile_len > 0) ? o->tile_len : 1024;
     if ((s->band_k != 4 && s->band_k != 8 && s->band_k != 16) || s->tile_len % 16 != 0) return fail(MP_ERR_BAD_ARG);
     s->refine = o ? (o->refine != 0) : 1;
-    s->accum64 = o ? (o->accum64 != 0) : 0;
+    s->accum64 = o ? (o->accum64 != 0) : 1;
     s->index_init = (o && o->index_init == 0) ? 0 : -1;
     s->excl = o ? o->exclusion : -1;
     const double tile_cells = 64.0 * s->band_k * s->tile_len;
