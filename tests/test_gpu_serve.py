@@ -42,7 +42,7 @@ def test_result_is_the_same_alone_and_in_the_mix():
     assert st["ticks"] >= 2 and st["completed"] == 61
     # and every request equals the FP32-reseeded self-join with the same tiling
     for (ts, m), (p, i, _) in list(zip(reqs, out))[::10]:
-        p1, i1, _ = tc.selfjoin_f32r(ts, m, band_k=8, tile_len=1024)
+        p1, i1, _ = tc.selfjoin_f32r(ts, m, band_k=8, tile_len=4096)
         assert np.array_equal(p, p1) and np.array_equal(i, i1)
 
 
@@ -70,7 +70,7 @@ def test_admission_by_hbm_footprint():
             ids = [srv.submit(ts, m) for ts, m in reqs]
             results = [srv.wait(r) for r in ids]
             for (ts, m), (p, i, _) in zip(reqs, results):
-                p1, i1, _ = tc.selfjoin_f32r(ts, m, band_k=8, tile_len=1024)
+                p1, i1, _ = tc.selfjoin_f32r(ts, m, band_k=8, tile_len=4096)
                 assert np.array_equal(p, p1)
             assert srv.stats()["hbm_used_bytes"] == 0
 

@@ -416,6 +416,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     desc.band_off = 0;
     desc.init_c1 = static_cast<float>(1.0 - 1.0 / static_cast<double>(m));
     const int64_t eprefix[2] = {0, g.P};
+    const int64_t sprefix[2] = {0, mp::stat_chunks(static_cast<int>(g.P))};
     const int64_t tprefix[2] = {0, ntiles};
 
     std::lock_guard<std::mutex> guard(c->lock);
@@ -424,7 +425,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n + 64, 8) +
                         3 * aligned_bytes(n + 64, 4) + aligned_bytes(n + 64, 8) +
                         (accum64 ? 2 * aligned_bytes(n + 64, 8) : 0) +
-                        aligned_bytes(1, sizeof(mp::SeriesDesc)) + 2 * aligned_bytes(2, 8) +
+                        aligned_bytes(1, sizeof(mp::SeriesDesc)) + 3 * aligned_bytes(2, 8) +
                         aligned_bytes(nbands + 1, 8) + aligned_bytes(1, 4);
     MP_TRY(c->work.reserve(need));
     c->work.used = 0;
@@ -439,6 +440,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     double* dg64 = accum64 ? c->work.take<double>(n + 64) : nullptr;
     auto* ddesc = c->work.take<mp::SeriesDesc>(1);
     auto* dep = c->work.take<int64_t>(2);
+    auto* dsp = c->work.take<int64_t>(2);
     auto* dtp = c->work.take<int64_t>(2);
     auto* dbp = c->work.take<int64_t>(nbands + 1);
     auto* counter = c->work.take<unsigned int>(1);
@@ -446,6 +448,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     MP_HIP(hipEventRecord(c->ev[0], st));
     MP_HIP(hipMemcpyAsync(ddesc, &desc, sizeof(desc), hipMemcpyHostToDevice, st));
     MP_HIP(hipMemcpyAsync(dep, eprefix, sizeof(eprefix), hipMemcpyHostToDevice, st));
+    MP_HIP(hipMemcpyAsync(dsp, sprefix, sizeof(sprefix), hipMemcpyHostToDevice, st));
     MP_HIP(hipMemcpyAsync(dtp, tprefix, sizeof(tprefix), hipMemcpyHostToDevice, st));
     MP_HIP(hipMemcpyAsync(dbp, bprefix.data(), sizeof(int64_t) * bprefix.size(),
                           hipMemcpyHostToDevice, st));
@@ -454,6 +457,8 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     mp::F32rBatch b{};
     b.descs = ddesc;
     b.entry_prefix = dep;
+    b.stat_prefix = dsp;
+    b.nstat = sprefix[1];
     b.tile_prefix = dtp;
     b.band_prefix = dbp;
     b.nseries = 1;

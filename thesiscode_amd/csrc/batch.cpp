@@ -59,6 +59,7 @@ int BatchPlan::build(int nseries_, const int64_t* lengths, const int32_t* window
     tile_len = tile_len_;
     descs.assign(nseries, SeriesDesc{});
     eprefix.assign(nseries + 1, 0);
+    sprefix.assign(nseries + 1, 0);
     tprefix.assign(nseries + 1, 0);
     bprefix.clear();
     total_n = 0;
@@ -91,6 +92,7 @@ int BatchPlan::build(int nseries_, const int64_t* lengths, const int32_t* window
         }
         total_n += n;
         eprefix[s + 1] = eprefix[s] + d.P;
+        sprefix[s + 1] = sprefix[s] + stat_chunks(d.P);
         tprefix[s + 1] = tprefix[s] + acc;
         cells += 0.5 * static_cast<double>(d.tl) * static_cast<double>(d.tl + 1);
     }
@@ -134,7 +136,7 @@ int upload_plan(mp_batch* b) {
     const int64_t N = p.total_n + 64;
     const size_t need = 2 * al(N, 8) + 3 * al(N, 4) + al(N, 8) + (b->accum64 ? 2 * al(N, 8) : 0) +
                         al(p.nseries, sizeof(mp::SeriesDesc)) +
-                        2 * al(p.nseries + 1, 8) + al(p.bprefix.size(), 8) + al(1, 4);
+                        3 * al(p.nseries + 1, 8) + al(p.bprefix.size(), 8) + al(1, 4);
     if (need > b->bytes) {
         if (b->mem) (void)hipFree(b->mem);
         b->mem = nullptr;
@@ -160,17 +162,21 @@ int upload_plan(mp_batch* b) {
     d.accum64 = b->accum64;
     auto* dd = reinterpret_cast<mp::SeriesDesc*>(take(al(p.nseries, sizeof(mp::SeriesDesc))));
     auto* dep = reinterpret_cast<int64_t*>(take(al(p.nseries + 1, 8)));
+    auto* dsp = reinterpret_cast<int64_t*>(take(al(p.nseries + 1, 8)));
     auto* dtp = reinterpret_cast<int64_t*>(take(al(p.nseries + 1, 8)));
     auto* dbp = reinterpret_cast<int64_t*>(take(al(p.bprefix.size(), 8)));
     d.counter = reinterpret_cast<unsigned int*>(take(al(1, 4)));
     MP_HIP(hipMemcpyAsync(dd, p.descs.data(), sizeof(mp::SeriesDesc) * p.nseries, hipMemcpyHostToDevice,
                           b->stream));
     MP_HIP(hipMemcpyAsync(dep, p.eprefix.data(), 8 * (p.nseries + 1), hipMemcpyHostToDevice, b->stream));
+    MP_HIP(hipMemcpyAsync(dsp, p.sprefix.data(), 8 * (p.nseries + 1), hipMemcpyHostToDevice, b->stream));
     MP_HIP(hipMemcpyAsync(dtp, p.tprefix.data(), 8 * (p.nseries + 1), hipMemcpyHostToDevice, b->stream));
     MP_HIP(hipMemcpyAsync(dbp, p.bprefix.data(), 8 * p.bprefix.size(), hipMemcpyHostToDevice, b->stream));
     MP_HIP(hipStreamSynchronize(b->stream));
     d.descs = dd;
     d.entry_prefix = dep;
+    d.stat_prefix = dsp;
+    d.nstat = p.sprefix[p.nseries];
     d.tile_prefix = dtp;
     d.band_prefix = dbp;
     d.nseries = p.nseries;

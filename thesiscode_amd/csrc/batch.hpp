@@ -14,6 +14,7 @@ struct BatchPlan {
     int band_k = 8, tile_len = 1024;
     std::vector<SeriesDesc> descs;
     std::vector<int64_t> eprefix;  // prefix of profile lengths
+    std::vector<int64_t> sprefix;  // prefix of statistics chunks
     std::vector<int64_t> tprefix;  // prefix of tiles per series
     std::vector<int64_t> bprefix;  // per series: prefix of tiles over its bands
     int64_t total_n = 0, total_P = 0, ntiles = 0;

@@ -446,7 +446,7 @@ __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(3))) voi
 }
 // K = 8 holds twice the row records; capping its registers would spill.
 template <int K, bool WRITEBACK>
-__global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(2))) void k_walk_f64_wide(MP_WALK_PARAMS) {
+__global__ __launch_bounds__(kGroup) void k_walk_f64_wide(MP_WALK_PARAMS) {
     walk_body<K, WRITEBACK>(MP_WALK_ARGS);
 }
 #undef MP_WALK_PARAMS

@@ -1,9 +1,10 @@
 // FP32-reseeded SCRIMP path (one series or a batch of series in one launch).
 //
 //   k_comp_stats   mean/sigma of every window from compensated (Neumaier,
-//                  exact-product) sliding sums in FP64, restarted every 256
-//                  windows from a direct sum, on data shifted by the chunk's
-//                  first value so that mean^2 >> sigma^2 does not cancel.
+//                  exact-product) sliding sums in FP64, restarted every 64
+//                  windows of a series from a direct sum, on data shifted by
+//                  the chunk's first value so that mean^2 >> sigma^2 does not
+//                  cancel.
 //   k_prep_terms   FP32 recurrence terms df, dg and 1/(sigma*sqrt(m)).
 //   k_tiles_f32    persistent tile kernel: a tile is a band of 64*K diagonals
 //                  times T columns.  Its start is seeded with an exact FP64
@@ -67,62 +68,44 @@ __device__ __forceinline__ int find_series(const int64_t* __restrict__ prefix, i
     return lo;
 }
 
-constexpr int kStatChunk = 256;
-
+// One thread per chunk of kStatChunk windows of one series: a direct
+// compensated sum at the chunk's first window, then sliding updates.  Chunks
+// start at series-local multiples of kStatChunk, so a series gets the same
+// statistics alone or in any batch.
 __global__ void k_comp_stats(const SeriesDesc* __restrict__ descs,
-                             const int64_t* __restrict__ entry_prefix, int nseries,
-                             int64_t nentries, const double* __restrict__ A,
-                             double* __restrict__ mu, double* __restrict__ sigma) {
-    const int64_t e0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kStatChunk;
-    if (e0 >= nentries) return;
-    const int64_t e1 = min(e0 + kStatChunk, nentries);
-    int64_t e = e0;
-    while (e < e1) {
-        const int s = find_series(entry_prefix, nseries, e);
-        const SeriesDesc d = descs[s];
-        const int64_t send = entry_prefix[s + 1];
-        const int i0 = static_cast<int>(e - entry_prefix[s]);
-        const int i1 = static_cast<int>(min(e1, send) - entry_prefix[s]);
-        const double* a = A + d.off;
-        const int m = d.m;
-        const double dm = static_cast<double>(m);
-        // The sums restart from a direct sum at every series-local multiple of
-        // kStatChunk and slide from there, so each window's statistics depend on
-        // the window alone, not on how a launch groups series into threads.
-        double shift = 0.0, S = 0.0, Sc = 0.0, Q = 0.0, Qc = 0.0;
-        auto restart = [&](int w) {
-            shift = a[w];
-            S = Sc = Q = Qc = 0.0;
-            for (int t = 0; t < m; ++t) {
-                const double x = a[w + t] - shift;
-                neu_add(S, Sc, x);
-                neu_add_sq(Q, Qc, x);
-            }
-        };
-        auto slide = [&](int w) {  // window w-1 -> window w
-            const double xo = a[w - 1] - shift, xn = a[w - 1 + m] - shift;
+                             const int64_t* __restrict__ stat_prefix, int nseries, int64_t nchunks,
+                             const double* __restrict__ A, double* __restrict__ mu,
+                             double* __restrict__ sigma) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nchunks) return;
+    const int s = find_series(stat_prefix, nseries, t);
+    const SeriesDesc d = descs[s];
+    const int i0 = static_cast<int>(t - stat_prefix[s]) * kStatChunk;
+    const int i1 = min(i0 + kStatChunk, d.P);
+    const double* a = A + d.off;
+    const int m = d.m;
+    const double dm = static_cast<double>(m);
+    // data shifted by the chunk's first value: sums of small numbers
+    const double shift = a[i0];
+    double S = 0.0, Sc = 0.0, Q = 0.0, Qc = 0.0;
+    for (int k = 0; k < m; ++k) {
+        const double x = a[i0 + k] - shift;
+        neu_add(S, Sc, x);
+        neu_add_sq(Q, Qc, x);
+    }
+    for (int i = i0; i < i1; ++i) {
+        if (i > i0) {  // window i-1 -> window i
+            const double xo = a[i - 1] - shift, xn = a[i - 1 + m] - shift;
             neu_add(S, Sc, xn);
             neu_add(S, Sc, -xo);
             neu_add_sq(Q, Qc, xn);
             neu_sub_sq(Q, Qc, xo);
-        };
-        const int anchor = i0 - i0 % kStatChunk;
-        restart(anchor);
-        for (int w = anchor + 1; w <= i0; ++w) slide(w);
-        for (int i = i0; i < i1; ++i) {
-            if (i > i0) {
-                if (i % kStatChunk == 0)
-                    restart(i);
-                else
-                    slide(i);
-            }
-            const double sm = (S + Sc) / dm;
-            double var = fma(-sm, sm, (Q + Qc) / dm);
-            if (!(var > 0.0)) var = 0.0;
-            mu[d.off + i] = shift + sm;
-            sigma[d.off + i] = sqrt(var);
         }
-        e = min(e1, send);
+        const double sm = (S + Sc) / dm;
+        double var = fma(-sm, sm, (Q + Qc) / dm);
+        if (!(var > 0.0)) var = 0.0;
+        mu[d.off + i] = shift + sm;
+        sigma[d.off + i] = sqrt(var);
     }
 }
 
@@ -479,11 +462,74 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
     }
 }
 
+__device__ __forceinline__ int find_stage(const StageDesc* d, int count, int64_t chunk) {
+    int lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (d[mid].chunk_base <= chunk)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// pinned series (then its band prefix, as int64 bits) -> device pools
+__global__ __launch_bounds__(256) void k_stage_in(const StageDesc* __restrict__ descs, int count,
+                                                  const double* __restrict__ pinned,
+                                                  double* __restrict__ A,
+                                                  int64_t* __restrict__ bandpool) {
+    const int64_t chunk = blockIdx.x;
+    const StageDesc d = descs[find_stage(descs, count, chunk)];
+    const int64_t e0 = (chunk - d.chunk_base) * kStageChunk;
+    const int64_t total = static_cast<int64_t>(d.n) + d.nb;
+    const int64_t e1 = min(e0 + kStageChunk, total);
+    const int64_t* pin_bands = reinterpret_cast<const int64_t*>(pinned + d.pin_off + d.n);
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        if (e < d.n)
+            A[d.dev_off + e] = pinned[d.pin_off + e];
+        else
+            bandpool[d.band_dev_off + (e - d.n)] = pin_bands[e - d.n];
+    }
+}
+
+// device profile/index -> pinned (profile, then index as int64 bits)
+__global__ __launch_bounds__(256) void k_stage_out(const StageDesc* __restrict__ descs, int count,
+                                                   const double* __restrict__ prof,
+                                                   const int64_t* __restrict__ idx,
+                                                   double* __restrict__ pinned) {
+    const int64_t chunk = blockIdx.x;
+    const StageDesc d = descs[find_stage(descs, count, chunk)];
+    const int64_t e0 = (chunk - d.chunk_base) * kStageChunk;
+    const int64_t e1 = min(e0 + kStageChunk, static_cast<int64_t>(d.n));
+    int64_t* pin_idx = reinterpret_cast<int64_t*>(pinned + d.pin_off + d.n);
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        pinned[d.pin_off + e] = prof[d.dev_off + e];
+        pin_idx[e] = idx[d.dev_off + e];
+    }
+}
+
+int launch_stage_in(const StageDesc* d, int count, int64_t nchunks, const double* pinned,
+                    double* A, int64_t* bandpool, hipStream_t st) {
+    if (count <= 0 || nchunks <= 0) return MP_OK;
+    hipLaunchKernelGGL(k_stage_in, dim3(static_cast<unsigned>(nchunks)), dim3(256), 0, st, d, count,
+                       pinned, A, bandpool);
+    return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
+}
+
+int launch_stage_out(const StageDesc* d, int count, int64_t nchunks, const double* prof,
+                     const int64_t* idx, double* pinned, hipStream_t st) {
+    if (count <= 0 || nchunks <= 0) return MP_OK;
+    hipLaunchKernelGGL(k_stage_out, dim3(static_cast<unsigned>(nchunks)), dim3(256), 0, st, d,
+                       count, prof, idx, pinned);
+    return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
+}
+
 // ---------------------------------------------------------------------------
 int launch_f32r_prep(const F32rBatch& b, hipStream_t st) {
-    const int64_t nchunks = (b.nentries + kStatChunk - 1) / kStatChunk;
-    hipLaunchKernelGGL(k_comp_stats, dim3(static_cast<unsigned>((nchunks + 127) / 128)), dim3(128),
-                       0, st, b.descs, b.entry_prefix, b.nseries, b.nentries, b.A, b.mu, b.sigma);
+    if (b.nstat > 0)
+        hipLaunchKernelGGL(k_comp_stats, dim3(static_cast<unsigned>((b.nstat + 63) / 64)), dim3(64), 0, st,
+                           b.descs, b.stat_prefix, b.nseries, b.nstat, b.A, b.mu, b.sigma);
     const unsigned g = static_cast<unsigned>((b.nentries + 255) / 256);
     hipLaunchKernelGGL(k_prep_terms, dim3(g), dim3(256), 0, st, b.descs, b.entry_prefix,
                        b.nseries, b.nentries, b.A, b.mu, b.sigma, b.df, b.dg, b.inv,

@@ -25,10 +25,11 @@ struct SeriesDesc {
 struct F32rBatch {
     const SeriesDesc* descs;    // device, nseries
     const int64_t* entry_prefix;  // device, nseries+1: prefix of P
+    const int64_t* stat_prefix;   // device, nseries+1: prefix of ceil(P / kStatChunk)
     const int64_t* tile_prefix;   // device, nseries+1: prefix of tiles
     const int64_t* band_prefix;   // device, concatenated per-series tile prefixes over bands
     int nseries;
-    int64_t nentries, ntiles;
+    int64_t nentries, ntiles, nstat;  // entries, tiles, statistics chunks
     int band_k, tile_len;
     const double* A;
     double *mu, *sigma;
@@ -38,6 +39,29 @@ struct F32rBatch {
     unsigned long long* keys;
     unsigned int* counter;
 };
+
+// Window statistics are computed in chunks of kStatChunk windows of one series;
+// each chunk restarts its sums at its first window, so the statistics of a
+// series do not depend on what else the launch carries.
+constexpr int kStatChunk = 64;
+inline int64_t stat_chunks(int P) { return (static_cast<int64_t>(P) + kStatChunk - 1) / kStatChunk; }
+
+// Staging of served requests between pinned host memory and the device pools:
+// one launch per tick moves every request's data (chunks of kStageChunk
+// elements, located by binary search over chunk_base).
+constexpr int kStageChunk = 4096;
+struct StageDesc {
+    int64_t pin_off;       // element offset in the pinned pool
+    int64_t dev_off;       // element offset in the device pools
+    int64_t band_dev_off;  // offset of the band prefix in the device band pool (stage in)
+    int64_t chunk_base;    // first chunk of this request in the launch
+    int n;                 // stage in: series length; stage out: profile length P
+    int nb;                // stage in: band prefix entries (stored after the series)
+};
+int launch_stage_in(const StageDesc* d, int count, int64_t nchunks, const double* pinned,
+                    double* A, int64_t* bandpool, hipStream_t st);
+int launch_stage_out(const StageDesc* d, int count, int64_t nchunks, const double* prof,
+                     const int64_t* idx, double* pinned, hipStream_t st);
 
 int launch_f32r_prep(const F32rBatch& b, hipStream_t st);
 int launch_f32r_tiles(const F32rBatch& b, int groups, hipStream_t st);

@@ -114,16 +114,26 @@ struct Request {
 constexpr int kSlots = 3;
 constexpr int kMaxRanges = 8192;
 
+// A tick's descriptor tables, laid out one after another in pinned memory and
+// moved to the device with one copy.
+struct TableArena {
+    char* h = nullptr;  // pinned host side
+    char* d = nullptr;  // device side, same layout
+    size_t cap = 0, used = 0;
+    template <typename T>
+    void take(size_t count, T** hp, T** dp) {
+        const size_t b = (count * sizeof(T) + 255) & ~size_t(255);
+        *hp = reinterpret_cast<T*>(h + used);
+        *dp = reinterpret_cast<T*>(d + used);
+        used += b;
+    }
+};
+
 struct Slot {
-    hipEvent_t tick_ev = nullptr, fin_ev = nullptr;
+    hipEvent_t tick_ev = nullptr, fin_ev = nullptr, prep_ev = nullptr;
     bool tick_busy = false, fin_busy = false;
-    // pinned host tables and their device copies
-    mp::SeriesDesc *h_tdesc = nullptr, *d_tdesc = nullptr;
-    int64_t *h_tpre = nullptr, *d_tpre = nullptr;
-    mp::SeriesDesc *h_pdesc = nullptr, *d_pdesc = nullptr;
-    int64_t *h_ppre = nullptr, *d_ppre = nullptr;
-    mp::SeriesDesc *h_fdesc = nullptr, *d_fdesc = nullptr;
-    int64_t *h_fpre = nullptr, *d_fpre = nullptr;
+    TableArena cmp;  // admission and tile tables (compute stream)
+    TableArena fin;  // finalize tables (finalize stream)
     std::vector<Request*> finishing;  // completes when fin_ev fires
 };
 
@@ -132,8 +142,13 @@ struct Slot {
 struct mp_server {
     int device = 0;
     int cus = 256;
-    hipStream_t cstream = nullptr, fstream = nullptr;
-    int band_k = 8, tile_len = 1024, refine = 1, excl = -1, accum64 = 1;
+    // Ticks alternate between two compute streams, so that a tick's tiles start
+    // while the previous tick's last tiles drain; each stream has its own work
+    // counter.  Set-up (statistics of admitted requests) stays in tick order.
+    hipStream_t cstream[2] = {nullptr, nullptr};
+    hipStream_t fstream = nullptr;
+    hipEvent_t last_prep_ev = nullptr, last_tick_ev = nullptr;
+    int band_k = 8, tile_len = 4096, refine = 1, excl = -1, accum64 = 1;
     int64_t index_init = -1;
     int64_t tick_tiles = 1;
     int64_t bytes_per_elem = 60;
@@ -147,7 +162,8 @@ struct mp_server {
     int64_t* bandpool = nullptr;
     unsigned int* counter = nullptr;
     RangeAlloc pool, bpool, ppool;
-    double* pinned = nullptr;  // pinned staging pool (doubles)
+    double* pinned = nullptr;      // pinned staging pool (doubles), host view
+    double* pinned_dev = nullptr;  // the same memory as the kernels address it
     Slot slots[kSlots];
     uint64_t tick_no = 0;
     // requests
@@ -169,6 +185,11 @@ struct mp_server {
 namespace {
 
 int64_t footprint_elems(int64_t n) { return n + 64; }
+// pinned staging of a request, in doubles: the series and its band prefix on the
+// way in, the profile and index on the way out
+int64_t pinned_need(const Request* r) {
+    return std::max<int64_t>(r->n + static_cast<int64_t>(r->bprefix.size()), 2 * static_cast<int64_t>(r->desc.P));
+}
 
 int build_request_plan(mp_server* s, Request* r) {
     const int m = r->m;
@@ -243,14 +264,15 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
         lk.lock();
         retire(s, false);
     }
-    hipStream_t cs = s->cstream;
+    hipStream_t cs = s->cstream[s->tick_no & 1];
+    unsigned int* counter = s->counter + 64 * (s->tick_no & 1);
     // ---- admission, in arrival order, while memory lasts ----
     std::vector<Request*> admitted;
     while (!s->waiting.empty() && static_cast<int>(admitted.size()) < kMaxRanges) {
         Request* r = s->waiting.front();
         const int64_t need = footprint_elems(r->n);
         const int64_t nb = r->desc.nbands + 1;
-        const int64_t pin_need = std::max<int64_t>(r->n, 2 * static_cast<int64_t>(r->desc.P));
+        const int64_t pin_need = pinned_need(r);
         const int64_t off = s->pool.alloc(need);
         if (off < 0) break;
         const int64_t boff = s->bpool.alloc(nb);
@@ -277,16 +299,23 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
         r->state = ACTIVE;
         r->t_admit = Clock::now();
         std::memcpy(s->pinned + poff, r->series.data(), sizeof(double) * r->n);
+        std::memcpy(s->pinned + poff + r->n, r->bprefix.data(), sizeof(int64_t) * r->bprefix.size());
         std::vector<double>().swap(r->series);
         admitted.push_back(r);
         s->active.push_back(r);
     }
     // ---- tile slices: every active request gets a share of this tick ----
-    std::vector<std::pair<Request*, int64_t>> take;
-    int64_t budget = s->tick_tiles;
+    // The tick budget grows with the backlog (up to 8x the base), so that an
+    // overloaded server spends less of its time on per-tick costs; an idle one
+    // keeps short ticks for latency.
     std::vector<Request*> with_tiles;
+    int64_t backlog = 0;
     for (Request* r : s->active)
-        if (r->state == ACTIVE && r->issued < r->ntiles) with_tiles.push_back(r);
+        if (r->state == ACTIVE && r->issued < r->ntiles) {
+            with_tiles.push_back(r);
+            backlog += r->ntiles - r->issued;
+        }
+    int64_t budget = std::min(8 * s->tick_tiles, std::max(s->tick_tiles, backlog / kSlots));
     if (static_cast<int>(with_tiles.size()) > kMaxRanges) with_tiles.resize(kMaxRanges);
     std::vector<int64_t> give(with_tiles.size(), 0);
     for (int pass = 0; pass < 2 && budget > 0; ++pass) {
@@ -303,18 +332,52 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
         }
     }
     int nranges = 0;
+    for (size_t k = 0; k < with_tiles.size(); ++k)
+        if (give[k] > 0) ++nranges;
+    // ---- this tick's tables, in one arena ----
+    sl.cmp.used = 0;
+    mp::StageDesc *h_sin = nullptr, *d_sin = nullptr;
+    mp::SeriesDesc *h_pdesc = nullptr, *d_pdesc = nullptr, *h_tdesc = nullptr, *d_tdesc = nullptr;
+    int64_t *h_ppre = nullptr, *d_ppre = nullptr, *h_tpre = nullptr, *d_tpre = nullptr;
+    int64_t *h_spre = nullptr, *d_spre = nullptr;
+    sl.cmp.take(admitted.size(), &h_sin, &d_sin);
+    sl.cmp.take(admitted.size(), &h_pdesc, &d_pdesc);
+    sl.cmp.take(admitted.size() + 1, &h_ppre, &d_ppre);
+    sl.cmp.take(admitted.size() + 1, &h_spre, &d_spre);
+    sl.cmp.take(static_cast<size_t>(nranges), &h_tdesc, &d_tdesc);
+    sl.cmp.take(static_cast<size_t>(nranges) + 1, &h_tpre, &d_tpre);
     int64_t ntiles = 0;
-    s->slots[s->tick_no % kSlots].h_tpre[0] = 0;
-    for (size_t k = 0; k < with_tiles.size(); ++k) {
-        if (give[k] <= 0) continue;
-        Request* r = with_tiles[k];
-        mp::SeriesDesc d = r->desc;
-        d.tile_base = r->issued;
-        sl.h_tdesc[nranges] = d;
-        ntiles += give[k];
-        sl.h_tpre[nranges + 1] = ntiles;
-        r->issued += give[k];
-        ++nranges;
+    {
+        int q = 0;
+        h_tpre[0] = 0;
+        for (size_t k = 0; k < with_tiles.size(); ++k) {
+            if (give[k] <= 0) continue;
+            Request* r = with_tiles[k];
+            mp::SeriesDesc d = r->desc;
+            d.tile_base = r->issued;
+            h_tdesc[q] = d;
+            ntiles += give[k];
+            h_tpre[q + 1] = ntiles;
+            r->issued += give[k];
+            ++q;
+        }
+    }
+    int64_t stage_chunks = 0;
+    h_ppre[0] = 0;
+    h_spre[0] = 0;
+    for (size_t k = 0; k < admitted.size(); ++k) {
+        Request* r = admitted[k];
+        h_pdesc[k] = r->desc;
+        h_ppre[k + 1] = h_ppre[k] + r->desc.P;
+        h_spre[k + 1] = h_spre[k] + mp::stat_chunks(r->desc.P);
+        mp::StageDesc& sd = h_sin[k];
+        sd.pin_off = r->pin_off;
+        sd.dev_off = r->dev_off;
+        sd.band_dev_off = r->band_off;
+        sd.chunk_base = stage_chunks;
+        sd.n = static_cast<int>(r->n);
+        sd.nb = static_cast<int>(r->bprefix.size());
+        stage_chunks += (r->n + sd.nb + mp::kStageChunk - 1) / mp::kStageChunk;
     }
     // requests whose every tile is now launched (or that have none) finish after this tick
     std::vector<Request*> finishing;
@@ -328,24 +391,21 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
     auto H = [&](hipError_t e) {
         if (e != hipSuccess && rc == MP_OK) rc = MP_ERR_DEVICE;
     };
+    // everything the previous tick set up must be in place before this tick's tiles
+    if (s->last_prep_ev) H(hipStreamWaitEvent(cs, s->last_prep_ev, 0));
+    if (sl.cmp.used > 0) H(hipMemcpyAsync(sl.cmp.d, sl.cmp.h, sl.cmp.used, hipMemcpyHostToDevice, cs));
     if (!admitted.empty()) {
-        sl.h_ppre[0] = 0;
-        for (size_t k = 0; k < admitted.size(); ++k) {
-            Request* r = admitted[k];
-            sl.h_pdesc[k] = r->desc;
-            sl.h_ppre[k + 1] = sl.h_ppre[k] + r->desc.P;
-            H(hipMemcpyAsync(s->A + r->dev_off, s->pinned + r->pin_off, sizeof(double) * r->n,
-                             hipMemcpyHostToDevice, cs));
-            H(hipMemcpyAsync(s->bandpool + r->band_off, r->bprefix.data(), sizeof(int64_t) * r->bprefix.size(),
-                             hipMemcpyHostToDevice, cs));
-        }
-        H(hipMemcpyAsync(sl.d_pdesc, sl.h_pdesc, sizeof(mp::SeriesDesc) * admitted.size(), hipMemcpyHostToDevice, cs));
-        H(hipMemcpyAsync(sl.d_ppre, sl.h_ppre, sizeof(int64_t) * (admitted.size() + 1), hipMemcpyHostToDevice, cs));
+        // one launch copies every admitted series and band prefix from pinned memory
+        if (rc == MP_OK)
+            rc = mp::launch_stage_in(d_sin, static_cast<int>(admitted.size()), stage_chunks, s->pinned_dev, s->A,
+                                     s->bandpool, cs);
         mp::F32rBatch pb{};
-        pb.descs = sl.d_pdesc;
-        pb.entry_prefix = sl.d_ppre;
+        pb.descs = d_pdesc;
+        pb.entry_prefix = d_ppre;
+        pb.stat_prefix = d_spre;
+        pb.nstat = h_spre[admitted.size()];
         pb.nseries = static_cast<int>(admitted.size());
-        pb.nentries = sl.h_ppre[admitted.size()];
+        pb.nentries = h_ppre[admitted.size()];
         pb.A = s->A;
         pb.mu = s->mu;
         pb.sigma = s->sigma;
@@ -358,13 +418,12 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
         pb.keys = s->keys;
         if (rc == MP_OK && pb.nentries > 0) rc = mp::launch_f32r_prep(pb, cs);
     }
+    H(hipEventRecord(sl.prep_ev, cs));
     if (nranges > 0) {
-        H(hipMemcpyAsync(sl.d_tdesc, sl.h_tdesc, sizeof(mp::SeriesDesc) * nranges, hipMemcpyHostToDevice, cs));
-        H(hipMemcpyAsync(sl.d_tpre, sl.h_tpre, sizeof(int64_t) * (nranges + 1), hipMemcpyHostToDevice, cs));
-        if (rc == MP_OK) rc = mp::launch_fill_u32(s->counter, 1, 0u, cs);
+        if (rc == MP_OK) rc = mp::launch_fill_u32(counter, 1, 0u, cs);
         mp::F32rBatch tb{};
-        tb.descs = sl.d_tdesc;
-        tb.tile_prefix = sl.d_tpre;
+        tb.descs = d_tdesc;
+        tb.tile_prefix = d_tpre;
         tb.band_prefix = s->bandpool;
         tb.nseries = nranges;
         tb.ntiles = ntiles;
@@ -379,7 +438,7 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
         tb.dg64 = s->dg64;
         tb.accum64 = s->accum64;
         tb.keys = s->keys;
-        tb.counter = s->counter;
+        tb.counter = counter;
         const int groups = static_cast<int>(std::min<int64_t>((ntiles + 3) / 4, 8 * s->cus));
         if (rc == MP_OK) rc = mp::launch_f32r_tiles(tb, groups, cs);
     }
@@ -387,30 +446,47 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
     // ---- finalize stream: unpack + copy out after this tick ----
     if (!finishing.empty()) {
         hipStream_t fs = s->fstream;
-        sl.h_fpre[0] = 0;
+        sl.fin.used = 0;
+        mp::SeriesDesc *h_fdesc = nullptr, *d_fdesc = nullptr;
+        int64_t *h_fpre = nullptr, *d_fpre = nullptr;
+        mp::StageDesc *h_sout = nullptr, *d_sout = nullptr;
+        sl.fin.take(finishing.size(), &h_fdesc, &d_fdesc);
+        sl.fin.take(finishing.size() + 1, &h_fpre, &d_fpre);
+        sl.fin.take(finishing.size(), &h_sout, &d_sout);
+        h_fpre[0] = 0;
+        int64_t out_chunks = 0;
         for (size_t k = 0; k < finishing.size(); ++k) {
-            sl.h_fdesc[k] = finishing[k]->desc;
-            sl.h_fpre[k + 1] = sl.h_fpre[k] + finishing[k]->desc.P;
+            Request* r = finishing[k];
+            h_fdesc[k] = r->desc;
+            h_fpre[k + 1] = h_fpre[k] + r->desc.P;
+            mp::StageDesc& sd = h_sout[k];
+            sd.pin_off = r->pin_off;
+            sd.dev_off = r->dev_off;
+            sd.band_dev_off = 0;
+            sd.chunk_base = out_chunks;
+            sd.n = r->desc.P;
+            sd.nb = 0;
+            out_chunks += (r->desc.P + mp::kStageChunk - 1) / mp::kStageChunk;
         }
-        H(hipMemcpyAsync(sl.d_fdesc, sl.h_fdesc, sizeof(mp::SeriesDesc) * finishing.size(), hipMemcpyHostToDevice, fs));
-        H(hipMemcpyAsync(sl.d_fpre, sl.h_fpre, sizeof(int64_t) * (finishing.size() + 1), hipMemcpyHostToDevice, fs));
+        H(hipMemcpyAsync(sl.fin.d, sl.fin.h, sl.fin.used, hipMemcpyHostToDevice, fs));
+        // a finishing request's tiles ran in this tick and possibly in the previous
+        // one, on the other compute stream
         H(hipStreamWaitEvent(fs, sl.tick_ev, 0));
+        if (s->last_tick_ev) H(hipStreamWaitEvent(fs, s->last_tick_ev, 0));
         mp::F32rBatch ub{};
-        ub.descs = sl.d_fdesc;
-        ub.entry_prefix = sl.d_fpre;
+        ub.descs = d_fdesc;
+        ub.entry_prefix = d_fpre;
         ub.nseries = static_cast<int>(finishing.size());
-        ub.nentries = sl.h_fpre[finishing.size()];
+        ub.nentries = h_fpre[finishing.size()];
         ub.A = s->A;
         ub.mu = s->mu;
         ub.sigma = s->sigma;
         ub.keys = s->keys;
         if (rc == MP_OK && ub.nentries > 0) rc = mp::launch_f32r_unpack(ub, s->refine, s->index_init, s->prof, s->idx, fs);
-        for (Request* r : finishing) {
-            const int64_t P = r->desc.P;
-            H(hipMemcpyAsync(s->pinned + r->pin_off, s->prof + r->dev_off, sizeof(double) * P, hipMemcpyDeviceToHost, fs));
-            H(hipMemcpyAsync(s->pinned + r->pin_off + P, s->idx + r->dev_off, sizeof(int64_t) * P,
-                             hipMemcpyDeviceToHost, fs));
-        }
+        // one launch copies every finished profile and index to pinned memory
+        if (rc == MP_OK)
+            rc = mp::launch_stage_out(d_sout, static_cast<int>(finishing.size()), out_chunks, s->prof, s->idx,
+                                      s->pinned_dev, fs);
         H(hipEventRecord(sl.fin_ev, fs));
     }
     lk.lock();
@@ -423,6 +499,8 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
                                        [](Request* r) { return r->state != ACTIVE; }),
                         s->active.end());
     }
+    s->last_prep_ev = sl.prep_ev;
+    s->last_tick_ev = sl.tick_ev;
     s->tick_no++;
     s->ticks++;
     return rc;
@@ -491,7 +569,7 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) s->cus = prop.multiProcessorCount;
     s->band_k = (o && o->band_k > 0) ? o->band_k : 8;
-    s->tile_len = (o && o->tile_len > 0) ? o->tile_len : 1024;
+    s->tile_len = (o && o->tile_len > 0) ? o->tile_len : 4096;
     if ((s->band_k != 4 && s->band_k != 8 && s->band_k != 16) || s->tile_len % 16 != 0) return fail(MP_ERR_BAD_ARG);
     s->refine = o ? (o->refine != 0) : 1;
     s->accum64 = o ? (o->accum64 != 0) : 1;
@@ -507,7 +585,7 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     if (cap < 1024) return fail(MP_ERR_BAD_ARG);
     const int64_t band_cap = cap / 64 + 1024;
     const size_t bytes = mp::aligned_bytes(cap, 8) * (s->accum64 ? 8 : 6) + mp::aligned_bytes(cap, 4) * 3 +
-                         mp::aligned_bytes(band_cap, 8) + 256;
+                         mp::aligned_bytes(band_cap, 8) + 512;
     if (hipMalloc(&s->dmem, bytes) != hipSuccess) return fail(MP_ERR_OOM);
     char* p = s->dmem;
     auto take = [&](size_t b) {
@@ -529,41 +607,42 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
         s->dg64 = reinterpret_cast<double*>(take(mp::aligned_bytes(cap, 8)));
     }
     s->bandpool = reinterpret_cast<int64_t*>(take(mp::aligned_bytes(band_cap, 8)));
-    s->counter = reinterpret_cast<unsigned int*>(take(256));
+    s->counter = reinterpret_cast<unsigned int*>(take(512));  // two counters, 256 bytes apart
     s->pool.reset(cap);
     s->bpool.reset(band_cap);
     // pinned staging: inputs and outputs of the requests in flight
     const int64_t pin_cap = std::min<int64_t>(2 * cap, int64_t(1) << 27);
-    if (hipHostMalloc(reinterpret_cast<void**>(&s->pinned), sizeof(double) * pin_cap) != hipSuccess)
+    // coherent: the staging kernels read and write it directly over the bus, and a
+    // recycled range must never be served from a stale device cache line
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->pinned), sizeof(double) * pin_cap,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
         return fail(MP_ERR_OOM);
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&s->pinned_dev), s->pinned, 0) != hipSuccess)
+        return fail(MP_ERR_DEVICE);
     s->ppool.reset(pin_cap);
-    if (hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking) != hipSuccess) return fail(MP_ERR_DEVICE);
+    for (auto& st : s->cstream)
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return fail(MP_ERR_DEVICE);
     if (hipStreamCreateWithFlags(&s->fstream, hipStreamNonBlocking) != hipSuccess) return fail(MP_ERR_DEVICE);
     for (auto& sl : s->slots) {
         if (hipEventCreateWithFlags(&sl.tick_ev, hipEventDisableTiming) != hipSuccess) return fail(MP_ERR_DEVICE);
+        if (hipEventCreateWithFlags(&sl.prep_ev, hipEventDisableTiming) != hipSuccess) return fail(MP_ERR_DEVICE);
         if (hipEventCreateWithFlags(&sl.fin_ev, hipEventDisableTiming) != hipSuccess) return fail(MP_ERR_DEVICE);
-        const size_t db = sizeof(mp::SeriesDesc) * kMaxRanges, pb = sizeof(int64_t) * (kMaxRanges + 1);
+        const size_t db = (sizeof(mp::SeriesDesc) * kMaxRanges + 255) & ~size_t(255);
+        const size_t pb = (sizeof(int64_t) * (kMaxRanges + 1) + 255) & ~size_t(255);
+        const size_t sb = (sizeof(mp::StageDesc) * kMaxRanges + 255) & ~size_t(255);
+        sl.cmp.cap = sb + 2 * db + 3 * pb;
+        sl.fin.cap = db + pb + sb;
         void* h = nullptr;
         void* d = nullptr;
-        if (hipHostMalloc(&h, 3 * (db + pb)) != hipSuccess) return fail(MP_ERR_OOM);
-        if (hipMalloc(&d, 3 * (db + pb)) != hipSuccess) {
+        if (hipHostMalloc(&h, sl.cmp.cap + sl.fin.cap) != hipSuccess) return fail(MP_ERR_OOM);
+        if (hipMalloc(&d, sl.cmp.cap + sl.fin.cap) != hipSuccess) {
             (void)hipHostFree(h);
             return fail(MP_ERR_OOM);
         }
-        char* hc = static_cast<char*>(h);
-        char* dc = static_cast<char*>(d);
-        sl.h_tdesc = reinterpret_cast<mp::SeriesDesc*>(hc);
-        sl.h_pdesc = reinterpret_cast<mp::SeriesDesc*>(hc + db);
-        sl.h_fdesc = reinterpret_cast<mp::SeriesDesc*>(hc + 2 * db);
-        sl.h_tpre = reinterpret_cast<int64_t*>(hc + 3 * db);
-        sl.h_ppre = reinterpret_cast<int64_t*>(hc + 3 * db + pb);
-        sl.h_fpre = reinterpret_cast<int64_t*>(hc + 3 * db + 2 * pb);
-        sl.d_tdesc = reinterpret_cast<mp::SeriesDesc*>(dc);
-        sl.d_pdesc = reinterpret_cast<mp::SeriesDesc*>(dc + db);
-        sl.d_fdesc = reinterpret_cast<mp::SeriesDesc*>(dc + 2 * db);
-        sl.d_tpre = reinterpret_cast<int64_t*>(dc + 3 * db);
-        sl.d_ppre = reinterpret_cast<int64_t*>(dc + 3 * db + pb);
-        sl.d_fpre = reinterpret_cast<int64_t*>(dc + 3 * db + 2 * pb);
+        sl.cmp.h = static_cast<char*>(h);
+        sl.cmp.d = static_cast<char*>(d);
+        sl.fin.h = sl.cmp.h + sl.cmp.cap;
+        sl.fin.d = sl.cmp.d + sl.cmp.cap;
     }
     s->engine = std::thread(engine_main, s);
     *out = s;
@@ -586,7 +665,7 @@ int mp_server_submit(mp_server* s, const double* ts, int64_t n, int32_t m, doubl
     if (s->failed != MP_OK) return s->failed;
     build_request_plan(s, r.get());
     if (footprint_elems(n) > s->pool.capacity() ||
-        std::max<int64_t>(n, 2 * static_cast<int64_t>(r->desc.P)) > s->ppool.capacity() ||
+        pinned_need(r.get()) > s->ppool.capacity() ||
         r->desc.nbands + 1 > s->bpool.capacity())
         return MP_ERR_ADMISSION;
     r->series.assign(ts, ts + n);
@@ -677,17 +756,20 @@ int mp_server_destroy(mp_server* s) {
         s->engine.join();
     }
     (void)hipSetDevice(s->device);
-    if (s->cstream) (void)hipStreamSynchronize(s->cstream);
+    for (auto st : s->cstream)
+        if (st) (void)hipStreamSynchronize(st);
     if (s->fstream) (void)hipStreamSynchronize(s->fstream);
     for (auto& sl : s->slots) {
         if (sl.tick_ev) (void)hipEventDestroy(sl.tick_ev);
+        if (sl.prep_ev) (void)hipEventDestroy(sl.prep_ev);
         if (sl.fin_ev) (void)hipEventDestroy(sl.fin_ev);
-        if (sl.h_tdesc) (void)hipHostFree(sl.h_tdesc);
-        if (sl.d_tdesc) (void)hipFree(sl.d_tdesc);
+        if (sl.cmp.h) (void)hipHostFree(sl.cmp.h);
+        if (sl.cmp.d) (void)hipFree(sl.cmp.d);
     }
     if (s->pinned) (void)hipHostFree(s->pinned);
     if (s->dmem) (void)hipFree(s->dmem);
-    if (s->cstream) (void)hipStreamDestroy(s->cstream);
+    for (auto st : s->cstream)
+        if (st) (void)hipStreamDestroy(st);
     if (s->fstream) (void)hipStreamDestroy(s->fstream);
     delete s;
     return MP_OK;
