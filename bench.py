@@ -232,6 +232,20 @@ def run_precision(workload: str, tiles=(1024, 4096, 8192)):
     return {"workload": workload, "n": n, "m": m, "f64_device_ms": t64["total_ms"], "variants": rows}
 
 
+def copy_probe(gib: int = 1, reps: int = 3):
+    """A plain device-to-device copy of known size, to scale rocprof's FETCH_SIZE and
+    WRITE_SIZE counters into bytes (run this under `rocprofv3 --pmc ...`)."""
+    import torch
+
+    n = gib << 30
+    x = torch.ones(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    return {"copy_bytes": n, "reps": reps}
+
+
 def cpu_baseline(workload: str, target_s: float = 15.0):
     """Oracle restatement of scrimp_triv_par on a bounded sample (every k-th block).
 
@@ -287,6 +301,8 @@ def main() -> int:
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
     ap.add_argument("--serve-tick-cells", type=float, default=4e9)
     ap.add_argument("--no-variant", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy 1 GiB device to device 3 times (calibration of rocprof byte counters)")
     ap.add_argument("--precision", action="store_true",
                     help="report the FP32-reseeded profile error against FP64 on --workload")
     args = ap.parse_args()
@@ -295,6 +311,9 @@ def main() -> int:
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.copy_probe:
+        print(json.dumps(copy_probe()), flush=True)
+        return 0
     if args.precision:
         print(json.dumps(run_precision(args.workload)), flush=True)
         return 0
