@@ -85,3 +85,20 @@ def test_tiny_geometries():
         p64, i64, _ = tc.selfjoin_f64(A, m)
         p, i, _ = tc.selfjoin_f32r(A, m)
         assert np.all(rel_err(p, p64, m) <= 1e-4), (n, m)
+
+
+def test_flat_windows_report_the_initial_score():
+    # Windows of zero variance have NaN scores in FP64 (1/sigma = inf), which
+    # never win: the FP64 path reports the initial score -1.0 (d^2 = 2m + 2)
+    # and the initial index for them.  The FP32 path does the same.
+    A = tc.random_walk(22, 3000)
+    A[1200:1400] = 0.0
+    m = 40
+    p, i, _ = tc.selfjoin_f32r(A, m)
+    flat = np.zeros(p.size, dtype=bool)
+    flat[1200:1400 - m + 1] = True
+    assert np.all(p[flat] == 2 * m + 2) and np.all(i[flat] == -1)
+    # rows whose FP64 partner is an ordinary window agree within tolerance
+    p64, i64, _ = tc.selfjoin_f64(A, m)
+    ok = ~flat & np.isfinite(p64) & (p64 < 2 * m)
+    assert np.all(rel_err(p[ok], p64[ok], m) <= 1e-4)

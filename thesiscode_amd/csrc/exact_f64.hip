@@ -222,7 +222,18 @@ struct ColTree {  // maximum of N consecutive cells starting at cell K0 (pairwis
 
 // WRITEBACK: store each diagonal's final QT (the block drop-in's tmpQ); the
 // self-join does not need it and skips the per-cell end-of-diagonal test.
-template <int K, bool WRITEBACK>
+// RING: the row record a lane takes in each step comes from a per-wave ring of
+// records in LDS instead of from its neighbour lane by DPP (two LDS reads in
+// place of eight DPP moves and eight fills per step).  The ring holds rows of
+// the band indexed by row - dbase modulo RR = BW + 2F: the rows the current
+// window reads and the next window's new ones; its first F slots are mirrored
+// after the end so that a lane's reads within a window never wrap.  The F new
+// rows of window w+1 reach the ring at the start of window w from a staging
+// array, which a direct-to-LDS load filled one window earlier: no register
+// holds them and no wait falls inside a window.  Lanes read slots 8 apart, so
+// the ring is padded by 16 bytes after every 8 slots (slot s at byte
+// 32 s + 16 (s / 8)): the 16 lanes of an LDS cycle then cover all 64 banks.
+template <int K, bool WRITEBACK, bool RING>
 __device__ __forceinline__ void walk_body(
     const WalkRec* __restrict__ colrec, const WalkRec* __restrict__ rowrec,
     double* qt, unsigned long long* rowval, unsigned long long* rowkey,
@@ -239,6 +250,28 @@ __device__ __forceinline__ void walk_body(
     // [0, F): rows leaving lane 0; lane l > 0 writes scratch slot F + l + step
     __shared__ double s_rv[kWavesPerGroup][2 * F + kWave];
     __shared__ int s_ri[kWavesPerGroup][2 * F + kWave];
+    constexpr int RR = BW + 2 * F;  // ring slots (the mirror adds F)
+    static_assert(RR % F == 0, "a window's new rows take F consecutive slots");
+    static_assert(RR % 8 == 0 && (K * 8) % 8 == 0, "ring bases stay multiples of 8 slots");
+    constexpr int kRingBytes = 32 * (RR + F) + 16 * ((RR + F) / 8);
+    char* ringb = nullptr;
+    WalkRec* stage = nullptr;
+    unsigned long long* filt = nullptr;  // [0, F): column results, [F, 2F): row results
+    if constexpr (RING) {
+        __shared__ __align__(16) char s_ring[kWavesPerGroup][kRingBytes];
+        __shared__ WalkRec s_stage[kWavesPerGroup][F];
+        __shared__ unsigned long long s_filt[kWavesPerGroup][2 * F];
+        ringb = &s_ring[threadIdx.x / kWave][0];
+        stage = &s_stage[threadIdx.x / kWave][0];
+        filt = &s_filt[threadIdx.x / kWave][0];
+    }
+    // byte offset of ring slot s, and the two 16-byte halves of a record there
+    auto ring_off = [](int s) __attribute__((always_inline)) { return 32 * s + 16 * (s >> 3); };
+    auto ring_put = [&](int s, const WalkRec& r) __attribute__((always_inline)) {
+        double2* p = reinterpret_cast<double2*>(ringb + ring_off(s));
+        p[0] = make_double2(r.a, r.am);
+        p[1] = make_double2(r.s, r.mu);
+    };
 
     const int wv = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
@@ -253,8 +286,6 @@ __device__ __forceinline__ void walk_body(
 
     // Flush the results of columns [jstart, jstart+cnt) and of the rows that
     // left lane 0 at those columns.  Row ids are stored as id+1 (0 = none).
-    // pc/pr: the current results of the window's columns and rows, read when
-    // the window starts so that the read has landed by the flush.
     // pc/pr: the current results of this window's columns and rows, read one
     // window ahead.  Results only grow, so a plain (possibly stale) read filters
     // out the partials that cannot win; a stale read only lets an unneeded
@@ -332,6 +363,20 @@ __device__ __forceinline__ void walk_body(
                 wm[k] = r.mu;
                 rv[k] = NEG;
                 ri[k] = 0;
+                if constexpr (RING) {  // rows 0 .. BW-1 of the band, slot = row - dbase
+                    ring_put(lid0 + k, r);
+                    if (lid0 + k < F) ring_put(RR + lid0 + k, r);
+                }
+            }
+            if constexpr (RING) {
+                // rows BW .. BW+F-1 of the band to the ring, rows BW+F .. BW+2F-1
+                // (the first window's new rows) to the staging array
+                const WalkRec r = rowrec[dbase + BW + (lane & (F - 1))];
+                const WalkRec r2 = rowrec[dbase + BW + F + (lane & (F - 1))];
+                if (lane < F) {
+                    ring_put(BW + lane, r);
+                    stage[lane] = r2;
+                }
             }
 
             // One column j = c0 + W for all K diagonals of every lane; W is the
@@ -339,17 +384,31 @@ __device__ __forceinline__ void walk_body(
             // the LDS offsets are constants.  TAIL: the window may end before
             // F steps.  Column and feed records are two linear scalar streams.
             WalkRec cur_c, cur_f;  // the next step's column and feed records
+            int rbase = 0;         // ring byte offset of this window's first refill row
+            // A step's column partial and the row leaving lane 0 are written to
+            // LDS by the next step, after its wait for the scalar records: the
+            // wait (lgkmcnt, which also counts LDS writes) then never waits for
+            // writes issued just before it.
+            double pcv = NEG, prv = NEG;
+            int pci = -1, pri = 0;
+            auto put_partials = [&](int w) __attribute__((always_inline)) {
+                cv_out[w * kWave] = pcv;
+                ci_out[w * kWave] = pci;
+                rv_out[w] = prv;
+                ri_out[w] = pri;
+            };
             auto step = [&](int c0, auto w_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
                 constexpr int W = decltype(w_tag)::value;
                 constexpr int S = W % K;
                 constexpr bool TAIL = decltype(tail_tag)::value;
                 const int j = c0 + W;
                 if (TAIL && j >= L) return false;
-                // records of this column and of the row entering lane 63 (a
-                // padding record past the triangle) were loaded one step ago;
-                // load the next step's now, so the scalar loads overlap a step
-                // Steps are not interleaved by the scheduler: that keeps each
-                // step's use of its records (and so the wait for them) after the
+                // Records of this column and of the row entering lane 63 (a
+                // padding record past the triangle; the ring path reads rows
+                // from LDS instead) were loaded one step ago; the next step's
+                // are loaded now, so the scalar loads overlap a step.  Steps
+                // are not interleaved by the scheduler: that keeps each step's
+                // use of its records (and so the wait for them) after the
                 // previous step's loads have had a whole step to land.
                 __builtin_amdgcn_sched_barrier(0);
                 const WalkRec c = cur_c, f = cur_f;
@@ -357,14 +416,32 @@ __device__ __forceinline__ void walk_body(
                 // records waits for every scalar load in flight: take the wait
                 // here (the empty asm needs the values), before the next step's
                 // loads are issued (its memory clobber keeps them below).
-                asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu), "s"(f.a), "s"(f.am),
-                             "s"(f.s), "s"(f.mu)
-                             : "memory");
+                if constexpr (RING) {
+                    asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu) : "memory");
+                } else {
+                    asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu), "s"(f.a), "s"(f.am),
+                                 "s"(f.s), "s"(f.mu)
+                                 : "memory");
+                }
                 cur_c = colrec[j + 1];
-                cur_f = rowrec[j + 1 + dbase + BW];
+                if constexpr (!RING) cur_f = rowrec[j + 1 + dbase + BW];
+                // the row this lane takes at the end of the step: row
+                // c0 + W + dbase + (lane+1)*K, which lane+1 finishes now; read
+                // before the step's LDS writes, so the read overlaps the step
+                WalkRec nr;
+                if constexpr (RING) {
+                    // slot rbase + W, rbase a multiple of 8: a constant offset
+                    const double2* p =
+                        reinterpret_cast<const double2*>(ringb + rbase + 32 * W + 16 * (W >> 3));
+                    const double2 lo = p[0], hi = p[1];
+                    nr.a = lo.x;
+                    nr.am = lo.y;
+                    nr.s = hi.x;
+                    nr.mu = hi.y;
+                }
+                if constexpr (W > 0) put_partials(W - 1);
                 __builtin_amdgcn_sched_barrier(0);  // issue them before the step's work
                 const double ahj = c.a, ahjm = c.am, shj = c.s, mhj = c.mu;
-                const double fa = f.a, fam = f.am, fs = f.s, fm = f.mu;
                 double sc[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -381,23 +458,72 @@ __device__ __forceinline__ void walk_body(
                     rv[sl] = dmax(rv[sl], sc[k]);
                 }
                 const ScoreId col = ColTree<K, K>::template reduce<0>(sc, lid0);
-                cv_out[W * kWave] = col.v;
-                ci_out[W * kWave] = col.id;
-                rv_out[W] = rv[S];
-                ri_out[W] = ri[S];
+                pcv = col.v;
+                pci = col.id;
+                prv = rv[S];
+                pri = ri[S];
                 rv[S] = shift_down_f64(rv[S], NEG);
                 ri[S] = shift_down_i32_zero(ri[S]);
-                wa[S] = shift_down_f64(wa[S], fa);
-                wam[S] = shift_down_f64(wam[S], fam);
-                ws[S] = shift_down_f64(ws[S], fs);
-                wm[S] = shift_down_f64(wm[S], fm);
+                if constexpr (RING) {
+                    wa[S] = nr.a;
+                    wam[S] = nr.am;
+                    ws[S] = nr.s;
+                    wm[S] = nr.mu;
+                } else {
+                    wa[S] = shift_down_f64(wa[S], f.a);
+                    wam[S] = shift_down_f64(wam[S], f.am);
+                    ws[S] = shift_down_f64(ws[S], f.s);
+                    wm[S] = shift_down_f64(wm[S], f.mu);
+                }
                 return true;
             };
             auto window = [&](int c0, auto tail_tag) __attribute__((always_inline)) {
+                if constexpr (RING) {
+                    // The next window's new rows, band rows c0 + F + BW + x, go
+                    // from the staging array to slots (c0 + F + BW + x) mod RR,
+                    // which held rows c0 - F + x: finished in the last window.
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane < F) {
+                        const WalkRec t = stage[lane];
+                        const int slot = (c0 + F + BW + lane) % RR;
+                        ring_put(slot, t);
+                        if (slot < F) ring_put(RR + slot, t);
+                    }
+                    // then the staging array takes band rows c0 + 2F + BW + x
+                    // (padding records past the triangle), 16 bytes per lane;
+                    // the staging read above has completed before the wave goes on
+                    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+                    if (lane < 2 * F) {
+                        const char* src = reinterpret_cast<const char*>(
+                                              rowrec + (dbase + c0 + 2 * F + BW + (lane >> 1))) +
+                                          16 * (lane & 1);
+                        __builtin_amdgcn_global_load_lds(
+                            (const __attribute__((address_space(1))) void*)(src),
+                            (__attribute__((address_space(3))) void*)(stage), 16, 0, 0);
+                        // the current results of this window's columns and rows,
+                        // for the flush's read filter (4 bytes per lane, clamped)
+                        const int jj = min(c0 + (lane >> 1), L - 1);
+                        const char* cs = reinterpret_cast<const char*>(colval + jj) + 4 * (lane & 1);
+                        const char* rs = reinterpret_cast<const char*>(rowval + jj + dbase) + 4 * (lane & 1);
+                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(cs),
+                                                         (__attribute__((address_space(3))) void*)(filt), 4, 0, 0);
+                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rs),
+                                                         (__attribute__((address_space(3))) void*)(filt + F), 4,
+                                                         0, 0);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    rbase = ring_off((c0 + lid0 + K) % RR);  // a byte offset
+                }
                 bool go = true;
                 [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
                     ((go = go && step(c0, std::integral_constant<int, W>{}, tail_tag)), ...);
                 }(std::make_integer_sequence<int, F>{});
+                // the window's last step writes no successor's partials
+                if constexpr (decltype(tail_tag)::value) {
+                    put_partials(min(F, L - c0) - 1);
+                } else {
+                    put_partials(F - 1);
+                }
             };
             // results of column c0 + (lane mod F) and of its row, clamped in bounds
             auto current = [&](int c0, uint64_t& pc, uint64_t& pr) __attribute__((always_inline)) {
@@ -406,23 +532,33 @@ __device__ __forceinline__ void walk_body(
                 pr = rowval[jj + dbase];
             };
             int c0 = 0;
-            uint64_t pc, pr;
-            current(0, pc, pr);
+            uint64_t pc = 0, pr = 0;
+            if constexpr (!RING) current(0, pc, pr);
             cur_c = colrec[0];
-            cur_f = rowrec[dbase + BW];
+            if constexpr (!RING) cur_f = rowrec[dbase + BW];
             // let the band's set-up loads land here, so the windows never wait on
             // vector memory (whose counter would also cover the flush's atomics)
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
             // whole flush windows: no control flow inside a window
             for (; c0 + F <= L; c0 += F) {
                 window(c0, std::false_type{});
-                flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
-                    current(c0 + F, a, b);
-                });
+                if constexpr (RING) {
+                    pc = filt[lane & (F - 1)];
+                    pr = filt[F + (lane & (F - 1))];
+                    flush(c0, F, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
+                } else {
+                    flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
+                        current(c0 + F, a, b);
+                    });
+                }
             }
             // the last partial window
             if (c0 < L) {
                 window(c0, std::true_type{});
+                if constexpr (RING) {
+                    pc = filt[lane & (F - 1)];
+                    pr = filt[F + (lane & (F - 1))];
+                }
                 flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
             }
         }
@@ -442,12 +578,13 @@ __device__ __forceinline__ void walk_body(
 template <int K, bool WRITEBACK>
 __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_f64(
     MP_WALK_PARAMS) {
-    walk_body<K, WRITEBACK>(MP_WALK_ARGS);
+    walk_body<K, WRITEBACK, false>(MP_WALK_ARGS);
 }
-// K = 8 holds twice the row records; capping its registers would spill.
+// K = 8 holds twice the row records (one wave per SIMD); its rows come from the
+// LDS ring, which the one group per CU this leaves has room for.
 template <int K, bool WRITEBACK>
 __global__ __launch_bounds__(kGroup) void k_walk_f64_wide(MP_WALK_PARAMS) {
-    walk_body<K, WRITEBACK>(MP_WALK_ARGS);
+    walk_body<K, WRITEBACK, true>(MP_WALK_ARGS);
 }
 #undef MP_WALK_PARAMS
 #undef MP_WALK_ARGS

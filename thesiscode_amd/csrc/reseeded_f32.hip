@@ -102,8 +102,11 @@ __global__ void k_comp_stats(const SeriesDesc* __restrict__ descs,
             neu_sub_sq(Q, Qc, xo);
         }
         const double sm = (S + Sc) / dm;
-        double var = fma(-sm, sm, (Q + Qc) / dm);
-        if (!(var > 0.0)) var = 0.0;
+        const double qm = (Q + Qc) / dm;
+        double var = fma(-sm, sm, qm);
+        // below the rounding noise of the sums the window is flat (sigma = 0):
+        // a constant window reached by sliding need not give exactly 0
+        if (!(var > 64.0 * __DBL_EPSILON__ * qm)) var = 0.0;
         mu[d.off + i] = shift + sm;
         sigma[d.off + i] = sqrt(var);
     }
