@@ -401,7 +401,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     std::vector<int64_t> bprefix(static_cast<size_t>(nbands) + 1, 0);
     for (int b = 0; b < nbands; ++b) {
         const int64_t L = static_cast<int64_t>(g.W) - static_cast<int64_t>(b) * BW;
-        bprefix[b + 1] = bprefix[b] + (L + T - 1) / T;
+        bprefix[b + 1] = bprefix[b] + mp::band_tiles(L, band_k, T);
     }
     const int64_t ntiles = bprefix[nbands];
     mp::SeriesDesc desc{};
@@ -423,8 +423,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     MP_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n + 64, 8) +
-                        3 * aligned_bytes(n + 64, 4) + aligned_bytes(n + 64, 8) +
-                        (accum64 ? 2 * aligned_bytes(n + 64, 8) : 0) +
+                        aligned_bytes(n + 64, mp::term_rec_bytes(accum64)) + aligned_bytes(n + 64, 8) +
                         aligned_bytes(1, sizeof(mp::SeriesDesc)) + 3 * aligned_bytes(2, 8) +
                         aligned_bytes(nbands + 1, 8) + aligned_bytes(1, 4);
     MP_TRY(c->work.reserve(need));
@@ -432,12 +431,9 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     double* A = c->work.take<double>(n + 64);
     double* mu = c->work.take<double>(n + 64);
     double* sigma = c->work.take<double>(n + 64);
-    float* df = c->work.take<float>(n + 64);
-    float* dg = c->work.take<float>(n + 64);
-    float* inv = c->work.take<float>(n + 64);
+    void* terms = accum64 ? static_cast<void*>(c->work.take<mp::TermRec<double>>(n + 64))
+                          : static_cast<void*>(c->work.take<mp::TermRec<float>>(n + 64));
     auto* keys = c->work.take<unsigned long long>(n + 64);
-    double* df64 = accum64 ? c->work.take<double>(n + 64) : nullptr;
-    double* dg64 = accum64 ? c->work.take<double>(n + 64) : nullptr;
     auto* ddesc = c->work.take<mp::SeriesDesc>(1);
     auto* dep = c->work.take<int64_t>(2);
     auto* dsp = c->work.take<int64_t>(2);
@@ -469,11 +465,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     b.A = A;
     b.mu = mu;
     b.sigma = sigma;
-    b.df = df;
-    b.dg = dg;
-    b.inv = inv;
-    b.df64 = df64;
-    b.dg64 = dg64;
+    b.terms = terms;
     b.accum64 = accum64;
     b.keys = keys;
     b.counter = counter;

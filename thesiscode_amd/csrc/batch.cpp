@@ -87,7 +87,7 @@ int BatchPlan::build(int nseries_, const int64_t* lengths, const int32_t* window
         bprefix.push_back(0);
         for (int b = 0; b < d.nbands; ++b) {
             const int64_t L = static_cast<int64_t>(d.tl) - static_cast<int64_t>(b) * BW;
-            acc += (L + tile_len - 1) / tile_len;
+            acc += band_tiles(L, band_k, tile_len);
             bprefix.push_back(acc);
         }
         total_n += n;
@@ -134,7 +134,7 @@ namespace {
 int upload_plan(mp_batch* b) {
     const mp::BatchPlan& p = b->plan;
     const int64_t N = p.total_n + 64;
-    const size_t need = 2 * al(N, 8) + 3 * al(N, 4) + al(N, 8) + (b->accum64 ? 2 * al(N, 8) : 0) +
+    const size_t need = 2 * al(N, 8) + al(N, mp::term_rec_bytes(b->accum64)) + al(N, 8) +
                         al(p.nseries, sizeof(mp::SeriesDesc)) +
                         3 * al(p.nseries + 1, 8) + al(p.bprefix.size(), 8) + al(1, 4);
     if (need > b->bytes) {
@@ -153,12 +153,8 @@ int upload_plan(mp_batch* b) {
     mp::F32rBatch& d = b->dev;
     d.mu = reinterpret_cast<double*>(take(al(N, 8)));
     d.sigma = reinterpret_cast<double*>(take(al(N, 8)));
-    d.df = reinterpret_cast<float*>(take(al(N, 4)));
-    d.dg = reinterpret_cast<float*>(take(al(N, 4)));
-    d.inv = reinterpret_cast<float*>(take(al(N, 4)));
+    d.terms = take(al(N, mp::term_rec_bytes(b->accum64)));
     d.keys = reinterpret_cast<unsigned long long*>(take(al(N, 8)));
-    d.df64 = b->accum64 ? reinterpret_cast<double*>(take(al(N, 8))) : nullptr;
-    d.dg64 = b->accum64 ? reinterpret_cast<double*>(take(al(N, 8))) : nullptr;
     d.accum64 = b->accum64;
     auto* dd = reinterpret_cast<mp::SeriesDesc*>(take(al(p.nseries, sizeof(mp::SeriesDesc))));
     auto* dep = reinterpret_cast<int64_t*>(take(al(p.nseries + 1, 8)));

@@ -543,6 +543,10 @@ __device__ __forceinline__ void walk_body(
             for (; c0 + F <= L; c0 += F) {
                 window(c0, std::false_type{});
                 if constexpr (RING) {
+                    // the direct-to-LDS loads of this window (filter values and
+                    // the next rows) must have landed: the compiler does not
+                    // order LDS reads after them
+                    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                     pc = filt[lane & (F - 1)];
                     pr = filt[F + (lane & (F - 1))];
                     flush(c0, F, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
@@ -556,6 +560,7 @@ __device__ __forceinline__ void walk_body(
             if (c0 < L) {
                 window(c0, std::true_type{});
                 if constexpr (RING) {
+                    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                     pc = filt[lane & (F - 1)];
                     pr = filt[F + (lane & (F - 1))];
                 }

@@ -116,9 +116,7 @@ __global__ void k_prep_terms(const SeriesDesc* __restrict__ descs,
                              const int64_t* __restrict__ entry_prefix, int nseries,
                              int64_t nentries, const double* __restrict__ A,
                              const double* __restrict__ mu, const double* __restrict__ sigma,
-                             float* __restrict__ df, float* __restrict__ dg,
-                             float* __restrict__ inv, double* __restrict__ df64,
-                             double* __restrict__ dg64) {
+                             void* __restrict__ terms, int accum64) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= nentries) return;
     const int s = find_series(entry_prefix, nseries, e);
@@ -127,18 +125,18 @@ __global__ void k_prep_terms(const SeriesDesc* __restrict__ descs,
     const double* a = A + d.off;
     const int64_t o = d.off + i;
     const double sg = sigma[o];
-    inv[o] = (sg > 0.0) ? static_cast<float>(1.0 / (sg * sqrt(static_cast<double>(d.m)))) : 0.0f;
+    const float iv = (sg > 0.0) ? static_cast<float>(1.0 / (sg * sqrt(static_cast<double>(d.m)))) : 0.0f;
     float f = 0.0f, g = 0.0f;
     if (i + 1 < d.P) {
         f = static_cast<float>(0.5 * (a[i + d.m] - a[i]));
         g = static_cast<float>((a[i + d.m] - mu[o + 1]) + (a[i] - mu[o]));
     }
-    df[o] = f;
-    dg[o] = g;
-    if (df64) {
+    if (accum64) {
         // the same FP32-rounded terms, widened: only the accumulation differs
-        df64[o] = static_cast<double>(f);
-        dg64[o] = static_cast<double>(g);
+        static_cast<TermRec<double>*>(terms)[o] =
+            TermRec<double>{static_cast<double>(f), static_cast<double>(g), iv, 0.0f, 0.0};
+    } else {
+        static_cast<TermRec<float>*>(terms)[o] = TermRec<float>{f, g, iv, 0.0f};
     }
 }
 
@@ -170,6 +168,22 @@ __device__ __forceinline__ double key_max(double a, double b) {
     return r;
 }
 
+// Maximum of a key over lanes l, l^16, l^32 and l^48, in every lane.  Each
+// gfx950 row swap leaves a lane with its own dword and the dword of lane l^32
+// (l^16) in two registers, in some order; the maximum does not depend on it.
+__device__ __forceinline__ double key_max_quarters(double x) {
+    auto pair = [](double v, auto swap) __attribute__((always_inline)) {
+        const unsigned lo = static_cast<unsigned>(__double2loint(v));
+        const unsigned hi = static_cast<unsigned>(__double2hiint(v));
+        const auto l = swap(lo);
+        const auto h = swap(hi);
+        return key_max(__hiloint2double(static_cast<int>(h[0]), static_cast<int>(l[0])),
+                       __hiloint2double(static_cast<int>(h[1]), static_cast<int>(l[1])));
+    };
+    x = pair(x, [](unsigned v) { return __builtin_amdgcn_permlane32_swap(v, v, false, false); });
+    return pair(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
+}
+
 template <typename Acc>
 __device__ __forceinline__ Acc shift_down_acc(Acc src, Acc fill);
 template <>
@@ -182,14 +196,22 @@ __device__ __forceinline__ double shift_down_acc<double>(double src, double fill
 }
 
 // Acc = float: FP32 recurrence with FP32 accumulation.  Acc = double: the same
-// FP32-rounded terms accumulated in FP64 (an FP64 FMA issues at the FP32 rate).
+// FP32-rounded terms accumulated in FP64.
+//
+// Per step the wave needs two uniform records: the column's terms and the
+// terms of the row entering lane 63.  They are scalar loads issued one step
+// ahead; scalar loads may return out of order, so the wait for them is a wait
+// for every scalar and LDS operation in flight, and the step takes it first,
+// before it issues anything.  The step's column partial and the row leaving
+// lane 0 are written to LDS by the next step, after that wait, so the wait
+// never covers LDS writes issued just before it.
 template <int K, typename Acc>
 __device__ __forceinline__ void tiles_body(
     const SeriesDesc* __restrict__ descs, const int64_t* __restrict__ tile_prefix,
     const int64_t* __restrict__ band_prefix, int nseries, const double* __restrict__ A,
-    const double* __restrict__ mu, const Acc* __restrict__ df, const Acc* __restrict__ dg,
-    const float* __restrict__ inv, unsigned long long* keys, unsigned int* counter,
-    int64_t ntiles, int T) {
+    const double* __restrict__ mu, const TermRec<Acc>* __restrict__ terms,
+    unsigned long long* keys, unsigned int* counter, int64_t ntiles, int T) {
+    using Rec = TermRec<Acc>;
     constexpr int F = kFlush;
     constexpr int BW = kWave * K;
     static_assert(F % K == 0, "flush period must be a multiple of K");
@@ -217,10 +239,17 @@ __device__ __forceinline__ void tiles_body(
         const int tl = d.tl;
         const int m = d.m;
         const int L = tl - dbase;
-        const int cb = q * T;
-        const int ce = min(cb + T, L);
+        const int lm = band_main_cols(L, K);
+        const int nmain = (lm + T - 1) / T;
+        const bool edge = q >= nmain;  // the band's last tile: columns [lm, L)
+        const int cb = edge ? lm : q * T;
+        const int ce = edge ? L : min(cb + T, lm);
         const int64_t roff = d.off + d.base;  // row (vertical) views
         const int64_t coff = d.off;           // column (horizontal) views
+        // The whole tile is specialised on EDGE (main tiles: whole windows,
+        // no cell tests), so that each specialisation holds its own registers.
+        auto run_tile = [&](auto edge_tag) __attribute__((always_inline)) {
+        constexpr bool EDGE = decltype(edge_tag)::value;
 
         // ---- FP64 seeds, strided over lanes for coalesced loads ----
         {
@@ -243,23 +272,19 @@ __device__ __forceinline__ void tiles_body(
             for (int k = 0; k < K; ++k) seeds[lane + kWave * k] = acc[k];
             __builtin_amdgcn_wave_barrier();
         }
-        // Row record: recurrence terms and 1/norm.  In the last tile of a band
-        // some cells lie past the triangle; there each row also carries a bias,
-        // 1 for rows of the triangle and -1e30 past its end, so that an invalid
-        // cell's key is negative and never beats a valid one (no per-cell test).
-        // Every other tile adds the constant 1.
+        // Row records: recurrence terms and 1/norm.  Rows past the triangle
+        // (row >= tl) are clamped to its last row: every cell of such a row lies
+        // outside the triangle, and the windows that reach them mask those cells.
         Acc cov[K], wdf[K], wdg[K];
-        float wiv[K], wbs[K];
+        float wiv[K];
         double rk[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             cov[k] = static_cast<Acc>(seeds[lid0 + k]);
-            const int row = cb + dbase + lid0 + k;
-            const int r = min(row, tl - 1);
-            wdf[k] = df[roff + r];
-            wdg[k] = dg[roff + r];
-            wiv[k] = inv[roff + r];
-            wbs[k] = (row < tl) ? 1.0f : -1e30f;
+            const Rec t = terms[roff + min(cb + dbase + lid0 + k, tl - 1)];
+            wdf[k] = t.df;
+            wdg[k] = t.dg;
+            wiv[k] = t.inv;
             rk[k] = 0.0;
         }
         __builtin_amdgcn_wave_barrier();
@@ -268,24 +293,30 @@ __device__ __forceinline__ void tiles_body(
         double* const rk_out = &s_rk[wv][(lane == 0) ? 0 : F + lane];
         double* const ck_out = &s_ck[wv][0][lane];
 
-        // pc/pr: the current keys of the window's columns and rows, read when
-        // the window starts so that the read has landed by the flush
-        auto flush = [&](int jstart, int cnt, unsigned long long pc, unsigned long long pr) {
+        // The filter: a partial is merged only if it beats the key read when
+        // the window started (lane x < F holds those of column c0 + x and of its
+        // row).  Keys only grow, so a stale read only lets an unneeded atomic
+        // through.
+        unsigned long long pc = 0, pr = 0;
+        auto load_filter = [&](int c0) __attribute__((always_inline)) {
+            const int jj = min(c0 + (lane & (F - 1)), ce - 1);
+            pc = keys[coff + jj];
+            pr = keys[roff + jj + dbase];
+        };
+        auto flush = [&](int jstart, int cnt) __attribute__((always_inline)) {
             __builtin_amdgcn_wave_barrier();
+            // lane (cs, qq) reduces a quarter of column cs's 64 partials (for
+            // cs >= cnt a stale quarter, whose result is not used)
             const int cs = lane & (F - 1), qq = lane / F;
-            double bk = 0.0;
-            if (cs < cnt) {
-#pragma unroll 4
-                for (int v = 0; v < kWave / 4; ++v) bk = fmax(bk, s_ck[wv][cs][qq * (kWave / 4) + v]);
-            }
+            const double* part = &s_ck[wv][cs][qq * (kWave / 4)];
+            double bk = part[0];
 #pragma unroll
-            for (int off = F; off < kWave; off <<= 1) bk = fmax(bk, __shfl_xor(bk, off));
+            for (int v = 1; v < kWave / 4; ++v) bk = key_max(bk, part[v]);
+            bk = key_max_quarters(bk);
             if (lane < cnt && __double2hiint(bk) > 0) {
                 const int jj = jstart + lane;
                 const int lid = __double2loint(bk);
                 const int64_t partner = static_cast<int64_t>(d.base) + jj + dbase + lid;
-                // keys only grow: a plain (possibly stale) read filters the
-                // partials that cannot win
                 const unsigned long long gk = global_key(bk, partner);
                 if (gk > pc) atomicMax(&keys[coff + jj], gk);
             }
@@ -300,84 +331,80 @@ __device__ __forceinline__ void tiles_body(
             }
             __builtin_amdgcn_wave_barrier();
         };
+        // The row entering lane 63 at column j, clamped like the others.
+        auto feed_row = [&](int j) __attribute__((always_inline)) { return min(j + dbase + BW, tl - 1); };
+        Rec cur_c, cur_f;  // the next step's column and feed records
+        double pck = 0.0, prk = 0.0;  // the last step's column partial and row exit
+        auto put_partials = [&](int w) __attribute__((always_inline)) {
+            ck_out[w * (kWave + 1)] = pck;
+            rk_out[w] = prk;
+        };
 
         // One column j = c0 + W of the band; W is the step's place in the flush
         // window, so the row slot (W mod K) and the LDS offsets are constants.
-        // BIASED: the tile reaches past the triangle (feed rows are clamped and
-        // carry the bias).  TAIL: the window may end before F steps.
-        auto step = [&](int c0, auto w_tag, auto biased_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
+        // EDGE: the window reaches past the triangle; a cell (j, lid) lies in it
+        // iff j + lid < L, and a cell outside gets the key -1 (never a winner:
+        // valid keys are corr + 1 >= 0).  TAIL: the window may end before F steps.
+        auto step = [&](int c0, auto w_tag, auto edge_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
             constexpr int W = decltype(w_tag)::value;
             constexpr int S = W % K;
-            constexpr bool BIASED = decltype(biased_tag)::value;
+            constexpr bool EDGE = decltype(edge_tag)::value;
             constexpr bool TAIL = decltype(tail_tag)::value;
             const int j = c0 + W;
             if (TAIL && j >= ce) return false;
-            const Acc dfh = df[coff + j], dgh = dg[coff + j];
-            const float ivh = inv[coff + j];
-            // row entering lane 63.  Unbiased tiles need no clamp: only the last
-            // step's feed can reach row tl (entry P of the series, inside the
-            // padded arrays) and that row is never used.
-            const int inext = j + dbase + BW;
-            const int ifeed = BIASED ? min(inext, tl - 1) : inext;
-            const Acc fdf = df[roff + ifeed], fdg = dg[roff + ifeed];
-            const float fiv = inv[roff + ifeed];
+            __builtin_amdgcn_sched_barrier(0);
+            const Rec c = cur_c, f = cur_f;
+            asm volatile("" ::"s"(c.df), "s"(c.dg), "s"(c.inv), "s"(f.df), "s"(f.dg), "s"(f.inv) : "memory");
+            cur_c = terms[coff + j + 1];
+            cur_f = terms[roff + feed_row(j + 1)];
+            if constexpr (W > 0) put_partials(W - 1);
+            __builtin_amdgcn_sched_barrier(0);
+            const Acc dfh = c.df, dgh = c.dg;
+            const float ivh = c.inv;
             double ck;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int sl = (S + k) % K;
-                const float bias = BIASED ? wbs[sl] : 1.0f;
-                const float c1 = fmaf(static_cast<float>(cov[k]) * wiv[sl], ivh, bias);
+                float c1 = fmaf(static_cast<float>(cov[k]) * wiv[sl], ivh, 1.0f);
+                if constexpr (EDGE) c1 = (lid0 + k < L - j) ? c1 : -1.0f;
                 const double key = __hiloint2double(__float_as_int(c1), lid0 + k);
                 cov[k] = fma(wdf[sl], dgh, cov[k]);
                 cov[k] = fma(dfh, wdg[sl], cov[k]);
                 rk[sl] = key_max(rk[sl], key);
                 ck = (k == 0) ? key : key_max(ck, key);
             }
-            ck_out[W * (kWave + 1)] = ck;
-            rk_out[W] = rk[S];
+            pck = ck;
+            prk = rk[S];
             {
                 const uint64_t rb = static_cast<uint64_t>(__double_as_longlong(rk[S]));
                 const int lo = shift_down_i32_zero(static_cast<int>(rb));
                 const int hi = shift_down_i32_zero(static_cast<int>(rb >> 32));
                 rk[S] = __hiloint2double(hi, lo);
             }
-            wdf[S] = shift_down_acc<Acc>(wdf[S], fdf);
-            wdg[S] = shift_down_acc<Acc>(wdg[S], fdg);
-            wiv[S] = shift_down_f32(wiv[S], fiv);
-            if (BIASED) wbs[S] = shift_down_f32(wbs[S], (inext < tl) ? 1.0f : -1e30f);
+            wdf[S] = shift_down_acc<Acc>(wdf[S], f.df);
+            wdg[S] = shift_down_acc<Acc>(wdg[S], f.dg);
+            wiv[S] = shift_down_f32(wiv[S], f.inv);
             return true;
         };
-        auto window = [&](int c0, auto biased_tag, auto tail_tag) __attribute__((always_inline)) {
+        auto window = [&](int c0, auto edge_tag, auto tail_tag) __attribute__((always_inline)) {
+            load_filter(c0);
             bool go = true;
             [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
-                ((go = go && step(c0, std::integral_constant<int, W>{}, biased_tag, tail_tag)), ...);
+                ((go = go && step(c0, std::integral_constant<int, W>{}, edge_tag, tail_tag)), ...);
             }(std::make_integer_sequence<int, F>{});
-        };
-        auto current = [&](int c0, unsigned long long& pc, unsigned long long& pr) __attribute__((always_inline)) {
-            const int jj = min(c0 + (lane & (F - 1)), ce - 1);
-            pc = keys[coff + jj];
-            pr = keys[roff + jj + dbase];
-        };
-        auto walk = [&](auto biased_tag) __attribute__((always_inline)) {
-            int c0 = cb;
-            for (; c0 + F <= ce; c0 += F) {
-                unsigned long long pc, pr;
-                current(c0, pc, pr);
-                window(c0, biased_tag, std::false_type{});
-                flush(c0, F, pc, pr);
-            }
-            if (c0 < ce) {
-                unsigned long long pc, pr;
-                current(c0, pc, pr);
-                window(c0, biased_tag, std::true_type{});
-                flush(c0, ce - c0, pc, pr);
+            // the window's last step has no successor to write its partials
+            if constexpr (decltype(tail_tag)::value) {
+                put_partials(min(F, ce - c0) - 1);
+            } else {
+                put_partials(F - 1);
             }
         };
-        // cell (j, lid) lies in the triangle iff j + lid < L
-        if (ce - 1 + BW - 1 < L)
-            walk(std::false_type{});
-        else
-            walk(std::true_type{});
+        cur_c = terms[coff + cb];
+        cur_f = terms[roff + feed_row(cb)];
+        for (int c0 = cb; c0 < ce; c0 += F) {
+            window(c0, edge_tag, edge_tag);
+            flush(c0, EDGE ? min(F, ce - c0) : F);
+        }
         // rows still inside the band at the tile's end: slot sl holds row
         // ce + dbase + lid0 + ((sl - (ce - cb)) mod K)
 #pragma unroll
@@ -390,16 +417,20 @@ __device__ __forceinline__ void tiles_body(
                 if (gk > keys[roff + i]) atomicMax(&keys[roff + i], gk);
             }
         }
+        };
+        if (edge)
+            run_tile(std::true_type{});
+        else
+            run_tile(std::false_type{});
     }
 }
 
 #define MP_TILE_PARAMS                                                                          \
     const SeriesDesc *__restrict__ descs, const int64_t *__restrict__ tile_prefix,              \
         const int64_t *__restrict__ band_prefix, int nseries, const double *__restrict__ A,     \
-        const double *__restrict__ mu, const Acc *__restrict__ df, const Acc *__restrict__ dg,  \
-        const float *__restrict__ inv, unsigned long long *keys, unsigned int *counter,         \
-        int64_t ntiles, int T
-#define MP_TILE_ARGS descs, tile_prefix, band_prefix, nseries, A, mu, df, dg, inv, keys, counter, ntiles, T
+        const double *__restrict__ mu, const TermRec<Acc> *__restrict__ terms,                  \
+        unsigned long long *keys, unsigned int *counter, int64_t ntiles, int T
+#define MP_TILE_ARGS descs, tile_prefix, band_prefix, nseries, A, mu, terms, keys, counter, ntiles, T
 
 // K <= 8: at most 128 VGPRs, so that 4 waves share each SIMD.
 template <int K, typename Acc>
@@ -535,8 +566,7 @@ int launch_f32r_prep(const F32rBatch& b, hipStream_t st) {
                            b.descs, b.stat_prefix, b.nseries, b.nstat, b.A, b.mu, b.sigma);
     const unsigned g = static_cast<unsigned>((b.nentries + 255) / 256);
     hipLaunchKernelGGL(k_prep_terms, dim3(g), dim3(256), 0, st, b.descs, b.entry_prefix,
-                       b.nseries, b.nentries, b.A, b.mu, b.sigma, b.df, b.dg, b.inv,
-                       b.accum64 ? b.df64 : nullptr, b.accum64 ? b.dg64 : nullptr);
+                       b.nseries, b.nentries, b.A, b.mu, b.sigma, b.terms, b.accum64);
     hipLaunchKernelGGL(k_init_keys, dim3(g), dim3(256), 0, st, b.descs, b.entry_prefix,
                        b.nseries, b.nentries, b.keys);
     return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
@@ -549,11 +579,13 @@ int launch_f32r_tiles(const F32rBatch& b, int groups, hipStream_t st) {
         if (b.accum64)                                                                          \
             hipLaunchKernelGGL((KERNEL<KK, double>), dim3(groups), dim3(kGroup), 0, st,         \
                                b.descs, b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu,     \
-                               b.df64, b.dg64, b.inv, b.keys, b.counter, b.ntiles, b.tile_len); \
+                               static_cast<const TermRec<double>*>(b.terms), b.keys, b.counter, \
+                               b.ntiles, b.tile_len);                                           \
         else                                                                                    \
             hipLaunchKernelGGL((KERNEL<KK, float>), dim3(groups), dim3(kGroup), 0, st,          \
                                b.descs, b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu,     \
-                               b.df, b.dg, b.inv, b.keys, b.counter, b.ntiles, b.tile_len);     \
+                               static_cast<const TermRec<float>*>(b.terms), b.keys, b.counter,  \
+                               b.ntiles, b.tile_len);                                           \
         break;
     switch (b.band_k) {
         MP_TILE_CASE(4, k_tiles_f32)

@@ -4,11 +4,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.hpp"
+
 namespace mp {
 
-// One self-join of a batch.  All per-series arrays (series A, mu, sigma, df,
-// dg, inv, keys, profile, index) live at element offset `off` of their
-// concatenated buffers.
+// One self-join of a batch.  All per-series arrays (series A, mu, sigma, terms,
+// keys, profile, index) live at element offset `off` of their concatenated
+// buffers.
 struct SeriesDesc {
     int64_t off;       // element offset of this series in every per-series buffer
     int64_t out_off;   // offset of its P entries in the profile/index outputs
@@ -33,12 +35,45 @@ struct F32rBatch {
     int band_k, tile_len;
     const double* A;
     double *mu, *sigma;
-    float *df, *dg, *inv;
-    double *df64, *dg64;   // FP64 copies of the terms when the covariance accumulates in FP64
+    void* terms;   // one TermRec<double> (accum64) or TermRec<float> per entry
     int accum64;
     unsigned long long* keys;
     unsigned int* counter;
 };
+
+// The recurrence terms of window i, read together: df = (A[i+m] - A[i]) / 2,
+// dg = (A[i+m] - mu[i+1]) + (A[i] - mu[i]) (both rounded to FP32; zero for the
+// last window) and inv = 1 / (sigma sqrt(m)) (0 for a flat window).  With FP64
+// accumulation the FP32-rounded terms are stored widened.  The tile kernel
+// loads one record per column and per row fed into its band.
+template <typename Acc>
+struct TermRec;
+template <>
+struct alignas(16) TermRec<float> {
+    float df, dg, inv, pad;
+};
+template <>
+struct alignas(32) TermRec<double> {
+    double df, dg;
+    float inv, pad;
+    double pad2;
+};
+inline size_t term_rec_bytes(int accum64) { return accum64 ? sizeof(TermRec<double>) : sizeof(TermRec<float>); }
+
+// Tiles of a band of L columns (the band's longest diagonal).  Columns
+// [0, band_main_cols) are cut into main tiles of at most T columns, made of
+// whole flush windows in which every cell lies in the triangle; the remaining
+// columns, where the triangle ends inside the band, form one edge tile whose
+// windows test each cell.
+__host__ __device__ inline int band_main_cols(int L, int band_k) {
+    const int lm = L - (kWave * band_k + kFlush - 2);
+    return lm > 0 ? (lm / kFlush) * kFlush : 0;
+}
+__host__ __device__ inline int64_t band_tiles(int64_t L, int band_k, int T) {
+    if (L <= 0) return 0;
+    const int lm = band_main_cols(static_cast<int>(L), band_k);
+    return (lm + T - 1) / T + 1;
+}
 
 // Window statistics are computed in chunks of kStatChunk windows of one series;
 // each chunk restarts its sums at its first window, so the statistics of a

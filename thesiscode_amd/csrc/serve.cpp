@@ -155,8 +155,7 @@ struct mp_server {
     // device pools (element-indexed)
     char* dmem = nullptr;
     double *A = nullptr, *mu = nullptr, *sigma = nullptr, *prof = nullptr;
-    float *df = nullptr, *dg = nullptr, *inv = nullptr;
-    double *df64 = nullptr, *dg64 = nullptr;
+    void* terms = nullptr;  // one term record per element
     unsigned long long* keys = nullptr;
     int64_t* idx = nullptr;
     int64_t* bandpool = nullptr;
@@ -209,7 +208,7 @@ int build_request_plan(mp_server* s, Request* r) {
     int64_t acc = 0;
     for (int b = 0; b < d.nbands; ++b) {
         const int64_t L = static_cast<int64_t>(d.tl) - static_cast<int64_t>(b) * BW;
-        acc += (L + s->tile_len - 1) / s->tile_len;
+        acc += mp::band_tiles(L, s->band_k, s->tile_len);
         r->bprefix.push_back(acc);
     }
     r->ntiles = acc;
@@ -409,11 +408,7 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
         pb.A = s->A;
         pb.mu = s->mu;
         pb.sigma = s->sigma;
-        pb.df = s->df;
-        pb.dg = s->dg;
-        pb.inv = s->inv;
-        pb.df64 = s->df64;
-        pb.dg64 = s->dg64;
+        pb.terms = s->terms;
         pb.accum64 = s->accum64;
         pb.keys = s->keys;
         if (rc == MP_OK && pb.nentries > 0) rc = mp::launch_f32r_prep(pb, cs);
@@ -431,11 +426,7 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
         tb.tile_len = s->tile_len;
         tb.A = s->A;
         tb.mu = s->mu;
-        tb.df = s->df;
-        tb.dg = s->dg;
-        tb.inv = s->inv;
-        tb.df64 = s->df64;
-        tb.dg64 = s->dg64;
+        tb.terms = s->terms;
         tb.accum64 = s->accum64;
         tb.keys = s->keys;
         tb.counter = counter;
@@ -578,13 +569,13 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     const double tile_cells = 64.0 * s->band_k * s->tile_len;
     const double tc = tick_cells > 0 ? static_cast<double>(tick_cells) : 4e9;
     s->tick_tiles = std::max<int64_t>(1, static_cast<int64_t>(tc / tile_cells));
-    // device pools: 60 bytes per point
-    const int64_t per_elem = 8 * 3 + 4 * 3 + 8 + 8 + 8 + (s->accum64 ? 16 : 0);
+    // device pools: A, mu, sigma, profile, keys, index and a term record per point
+    const int64_t per_elem = 8 * 6 + static_cast<int64_t>(mp::term_rec_bytes(s->accum64));
     s->bytes_per_elem = per_elem;
     const int64_t cap = static_cast<int64_t>(hbm_budget_bytes / per_elem);
     if (cap < 1024) return fail(MP_ERR_BAD_ARG);
     const int64_t band_cap = cap / 64 + 1024;
-    const size_t bytes = mp::aligned_bytes(cap, 8) * (s->accum64 ? 8 : 6) + mp::aligned_bytes(cap, 4) * 3 +
+    const size_t bytes = mp::aligned_bytes(cap, 8) * 6 + mp::aligned_bytes(cap, mp::term_rec_bytes(s->accum64)) +
                          mp::aligned_bytes(band_cap, 8) + 512;
     if (hipMalloc(&s->dmem, bytes) != hipSuccess) return fail(MP_ERR_OOM);
     char* p = s->dmem;
@@ -599,13 +590,7 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     s->prof = reinterpret_cast<double*>(take(mp::aligned_bytes(cap, 8)));
     s->keys = reinterpret_cast<unsigned long long*>(take(mp::aligned_bytes(cap, 8)));
     s->idx = reinterpret_cast<int64_t*>(take(mp::aligned_bytes(cap, 8)));
-    s->df = reinterpret_cast<float*>(take(mp::aligned_bytes(cap, 4)));
-    s->dg = reinterpret_cast<float*>(take(mp::aligned_bytes(cap, 4)));
-    s->inv = reinterpret_cast<float*>(take(mp::aligned_bytes(cap, 4)));
-    if (s->accum64) {
-        s->df64 = reinterpret_cast<double*>(take(mp::aligned_bytes(cap, 8)));
-        s->dg64 = reinterpret_cast<double*>(take(mp::aligned_bytes(cap, 8)));
-    }
+    s->terms = take(mp::aligned_bytes(cap, mp::term_rec_bytes(s->accum64)));
     s->bandpool = reinterpret_cast<int64_t*>(take(mp::aligned_bytes(band_cap, 8)));
     s->counter = reinterpret_cast<unsigned int*>(take(512));  // two counters, 256 bytes apart
     s->pool.reset(cap);
