@@ -71,6 +71,25 @@ def test_band_width_variants_agree(band_k, accum64):
     assert np.array_equal(p, p_ref)
 
 
+@pytest.mark.parametrize("band_k", [4, 8, 16])
+@pytest.mark.parametrize("tile_len", [16, 48, 512])
+def test_tile_split_extremes(band_k, tile_len):
+    # A band is cut into main tiles of whole 16-column windows that lie inside
+    # the triangle and one edge tile where the triangle ends.  Cover main tiles
+    # of a single window (T=16), tile lengths that are odd multiples of 16, and
+    # series whose bands are shorter than 64K + 16 columns (edge tiles only) or
+    # just longer.  A column or row left out of every tile would show up as a
+    # worse profile value than the exact path's.
+    bw = 64 * band_k
+    for n, m in ((bw + 40, 16), (2 * bw + 300, 32), (7000, 64)):
+        A = tc.random_walk(n + band_k + tile_len, n)
+        p64, i64, _ = tc.selfjoin_f64(A, m)
+        p, i, tm = tc.selfjoin_f32r(A, m, band_k=band_k, tile_len=tile_len)
+        assert tm["tile_len"] == tile_len and tm["band_k"] == band_k
+        assert np.all(rel_err(p, p64, m) <= 1e-4), (n, m)
+        assert np.all(np.abs(np.arange(p.size) - i) > m // 4), (n, m)
+
+
 def test_winner_distance_is_recomputed_exactly():
     A = tc.random_walk(72, 5000)
     m = 50
