@@ -204,7 +204,7 @@ def run_serve_bench(seconds: float, rate: float, budget_gb: float, tick_cells: f
             "queue_p99_ms": float(np.percentile(q_v, 99)), "ticks": st["ticks"]}
 
 
-def run_precision(workload: str, tiles=(1024, 4096, 8192)):
+def run_precision(workload: str, tiles=(256, 1024, 4096, 8192)):
     """Profile error of the FP32-reseeded path against the exact FP64 path on one workload.
 
     rel = |d2 - d2_f64| / max(d2_f64, 2m * 1e-3); index "equal" is exact equality with

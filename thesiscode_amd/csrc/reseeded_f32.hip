@@ -251,25 +251,59 @@ __device__ __forceinline__ void tiles_body(
         auto run_tile = [&](auto edge_tag) __attribute__((always_inline)) {
         constexpr bool EDGE = decltype(edge_tag)::value;
 
-        // ---- FP64 seeds, strided over lanes for coalesced loads ----
+        // ---- FP64 seeds on the matrix cores ----
+        // seed[d] = sum_t A[r0 + d + t] * c[t] with c[t] = A[cb + t] - mu[cb]
+        // (centred column window), for the band's diagonals d = 256h + 16 d1 +
+        // d0.  With p = 4s + t0 and c = 0 outside [0, m):
+        //   D_h[d0][d1] = sum_s sum_t0 A[r0 + 256h + d0 + 4s + t0] * c[4s + t0 - 16 d1],
+        // a sum over s of 16x4 by 4x16 FP64 products whose first operand does
+        // not depend on d1 and second not on d0 (v_mfma_f64_16x16x4f64; lane l
+        // supplies A[l % 16][l / 16] and B[l / 16][l % 16], and holds
+        // D[4r + l / 16][l % 16] in its r-th result).  Element indices past the
+        // last one a valid diagonal reads are clamped: they only reach the
+        // sums of diagonals that lie outside the triangle.
         {
-            double acc[K];
+            typedef double v4d __attribute__((ext_vector_type(4)));
+            constexpr int H = K / 4;  // blocks of 256 diagonals
+            static_assert(K % 4 == 0, "bands are whole 256-diagonal blocks");
+            v4d acc[H];
 #pragma unroll
-            for (int k = 0; k < K; ++k) acc[k] = 0.0;
+            for (int h = 0; h < H; ++h) acc[h] = v4d{0.0, 0.0, 0.0, 0.0};
             const double muc = mu[coff + cb];
             const double* av = A + roff;
             const double* ah = A + coff + cb;
-            int rows[K];
+            const int r0 = cb + dbase;
+            const int lim = tl + m - 2;  // last element of the last valid row's window
+            const int l16 = lane % 16, t0 = lane / 16;
+            // Steps in blocks of U, all loads of a block issued before its
+            // products.  Steps past (m + 240) / 4 read columns past m, whose
+            // operand is 0, so the step count is rounded up to whole blocks.
+            constexpr int U = 8;
+            const int nsteps = ((m + 15 * 16 + 3) / 4 + U - 1) / U * U;
+            for (int st0 = 0; st0 < nsteps; st0 += U) {
+                double bv[U], a[U][H];
 #pragma unroll
-            for (int k = 0; k < K; ++k) rows[k] = min(cb + dbase + lane + kWave * k, tl - 1);
-            for (int t = 0; t < m; ++t) {
-                const double c = ah[t] - muc;
+                for (int u = 0; u < U; ++u) {
+                    const int st = st0 + u;
+                    const int ib = 4 * st + t0 - 16 * l16;
+                    const double raw = ah[min(max(ib, 0), m - 1)];
+                    bv[u] = (ib >= 0 && ib < m) ? raw - muc : 0.0;
 #pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] = fma(av[rows[k] + t], c, acc[k]);
+                    for (int h = 0; h < H; ++h) a[u][h] = av[min(r0 + 256 * h + l16 + 4 * st + t0, lim)];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+#pragma unroll
+                    for (int h = 0; h < H; ++h)
+                        acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][h], bv[u], acc[h], 0, 0, 0);
+                }
             }
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int k = 0; k < K; ++k) seeds[lane + kWave * k] = acc[k];
+            for (int h = 0; h < H; ++h) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) seeds[256 * h + 16 * l16 + 4 * r + t0] = acc[h][r];
+            }
             __builtin_amdgcn_wave_barrier();
         }
         // Row records: recurrence terms and 1/norm.  Rows past the triangle
