@@ -181,13 +181,16 @@ __global__ void k_pack_rec(const double* __restrict__ A, const double* __restric
 }
 
 // (value, local diagonal id) merge: the larger score wins; exact ties keep the
-// larger id so that merging is symmetric and deterministic.  NaN never wins.
+// larger id so that merging is symmetric and deterministic.  NaN (a masked
+// cell) ranks below every number, so a NaN operand never absorbs a score and
+// the merge is a maximum over a total order: any reduction order gives the
+// same result.
 struct ScoreId {
     double v;
     int id;
 };
 __device__ __forceinline__ ScoreId better(ScoreId a, ScoreId b) {
-    const bool take = b.v > a.v || (b.v == a.v && b.id > a.id);
+    const bool take = b.v > a.v || (a.v != a.v) || (b.v == a.v && b.id > a.id);
     return take ? b : a;
 }
 // Maximum of two scores with the plain instruction: it returns the other
@@ -204,6 +207,21 @@ __device__ __forceinline__ double dmax(double a, double b) {
 __device__ __forceinline__ ScoreId merge_left(ScoreId a, ScoreId b) {
     const double v = dmax(a.v, b.v);
     return ScoreId{v, (a.v == v) ? a.id : b.id};
+}
+
+// better() over lanes l, l^16, l^32 and l^48, in every lane.  Each gfx950 row
+// swap leaves a lane with its own dword and the dword of lane l^32 (l^16) in
+// two registers, in some order; better() is symmetric, so the order is free.
+__device__ __forceinline__ ScoreId better_quarters(ScoreId x) {
+    auto pair = [](ScoreId v, auto swap) __attribute__((always_inline)) {
+        const auto lo = swap(static_cast<unsigned>(__double2loint(v.v)));
+        const auto hi = swap(static_cast<unsigned>(__double2hiint(v.v)));
+        const auto id = swap(static_cast<unsigned>(v.id));
+        return better(ScoreId{__hiloint2double(static_cast<int>(hi[0]), static_cast<int>(lo[0])), static_cast<int>(id[0])},
+                      ScoreId{__hiloint2double(static_cast<int>(hi[1]), static_cast<int>(lo[1])), static_cast<int>(id[1])});
+    };
+    x = pair(x, [](unsigned v) { return __builtin_amdgcn_permlane32_swap(v, v, false, false); });
+    return pair(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
 }
 
 template <int K, int N>
@@ -242,11 +260,12 @@ __device__ __forceinline__ void walk_body(
     constexpr int F = kFlush;
     constexpr int BW = kWave * K;
     static_assert(F % K == 0, "flush period must be a multiple of K");
-    // Column partials, one row of 64 per step.  Unpadded (53,760 bytes of LDS
-    // per group, so that 3 groups fit in a CU's 160 KiB); the flush reads each
-    // row in a lane-rotated order instead, which spreads the banks.
-    __shared__ double s_cv[kWavesPerGroup][F][kWave];
-    __shared__ int s_ci[kWavesPerGroup][F][kWave];
+    // Column partials, one row of 64 per step, padded by one entry: a flush
+    // lane reads 16 consecutive entries of one row (immediate offsets), and
+    // the 16 lanes of an LDS phase then fall on distinct banks.  54,528 bytes
+    // of LDS per group, so that 3 groups still fit in a CU's 160 KiB.
+    __shared__ double s_cv[kWavesPerGroup][F][kWave + 1];
+    __shared__ int s_ci[kWavesPerGroup][F][kWave + 1];
     // [0, F): rows leaving lane 0; lane l > 0 writes scratch slot F + l + step
     __shared__ double s_rv[kWavesPerGroup][2 * F + kWave];
     __shared__ int s_ri[kWavesPerGroup][2 * F + kWave];
@@ -296,26 +315,42 @@ __device__ __forceinline__ void walk_body(
     auto flush = [&](int jstart, int cnt, int dbase, uint64_t& pc, uint64_t& pr, auto reload)
         __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
+        // Lane (cs, qq) holds a quarter of column cs's 64 partials.  A column
+        // result passes the filter iff its score reaches thr, the score of the
+        // truncated current value (the order map is monotone), so a column
+        // can pass only if one of its partials does.  The reduction (a
+        // pairwise tree, then the other quarters; better() is a maximum over
+        // (score, id), so the order is free) runs only when some partial of
+        // the window passes.  For cs >= cnt the quarter is stale and unused.
         const int cs = lane & (F - 1), qq = lane / F;
-        ScoreId b{NEG, -1};
-        if (cs < cnt) {
-            // better() is a maximum over (score, id), so the order is free
-#pragma unroll 2
-            for (int u = 0; u < kWave / 4; ++u) {
-                const int e = qq * (kWave / 4) + ((u + cs) & (kWave / 4 - 1));
-                const ScoreId c{s_cv[wv][cs][e], s_ci[wv][cs][e]};
-                b = better(b, c);
-            }
-        }
+        const double* const pv = &s_cv[wv][cs][qq * (kWave / 4)];
+        const int* const pi = &s_ci[wv][cs][qq * (kWave / 4)];
+        double v[kWave / 4];
 #pragma unroll
-        for (int off = F; off < kWave; off <<= 1) {
-            const ScoreId o{__shfl_xor(b.v, off), __shfl_xor(b.id, off)};
-            b = better(b, o);
-        }
+        for (int u = 0; u < kWave / 4; ++u) v[u] = pv[u];
+        const double thr = unord_f64((pc >> ib) << ib);
+        bool cand = false;
+#pragma unroll
+        for (int u = 0; u < kWave / 4; ++u) cand |= (v[u] >= thr);
+        cand = cand && (cs < cnt);
         const int jj = jstart + lane;
-        const uint64_t oc = ord_f64(b.v);
+        uint64_t oc = 0;
+        ScoreId b{NEG, -1};
+        if (__builtin_amdgcn_ballot_w64(cand) != 0) {
+            ScoreId c[kWave / 4];
+#pragma unroll
+            for (int u = 0; u < kWave / 4; ++u) c[u] = ScoreId{v[u], pi[u]};
+#pragma unroll
+            for (int w = kWave / 8; w >= 1; w >>= 1) {
+#pragma unroll
+                for (int u = 0; u < w; ++u) c[u] = better(c[u], c[u + w]);
+            }
+            b = better_quarters(c[0]);
+            oc = ord_f64(b.v);
+        }
         // (non-short-circuit forms: both tests are done before the reload)
-        const bool do_col = (lane < cnt) & (b.id >= 0) & ((oc >> ib) >= (pc >> ib));
+        // a NaN result: every partial of the column was a masked cell
+        const bool do_col = (lane < cnt) & (b.id >= 0) & (b.v == b.v) & ((oc >> ib) >= (pc >> ib));
         const double rvl = s_rv[wv][lane & (F - 1)];
         const int rid = s_ri[wv][lane & (F - 1)] - 1;
         const uint64_t orw = ord_f64(rvl);
@@ -392,8 +427,8 @@ __device__ __forceinline__ void walk_body(
             double pcv = NEG, prv = NEG;
             int pci = -1, pri = 0;
             auto put_partials = [&](int w) __attribute__((always_inline)) {
-                cv_out[w * kWave] = pcv;
-                ci_out[w * kWave] = pci;
+                cv_out[w * (kWave + 1)] = pcv;
+                ci_out[w * (kWave + 1)] = pci;
                 rv_out[w] = prv;
                 ri_out[w] = pri;
             };
