@@ -109,7 +109,7 @@ int geometry(int64_t n, int32_t m, const mp_opts* o, Geometry* g) {
 int pick_band_k_f64(const mp_ctx* c, int W, int requested) {
     if (requested > 0) return requested;
     static const int ks[4] = {1, 2, 4, 8};
-    static const double cost[4] = {150.0, 73.0, 44.0, 41.0};
+    static const double cost[4] = {134.0, 65.0, 37.0, 34.0};
     const int64_t simds = 4LL * std::max(1, c->cus);
     int best = 4;
     double best_t = 0.0;
