@@ -155,15 +155,17 @@ int mp_batch_destroy(mp_batch* batch);
 
 /* ---- serving: continuous batching of self-join requests --------------------
  * mp_server_create starts an engine thread on the context's device with an
- * HBM budget for resident request state (60 bytes per point: series,
- * statistics, recurrence terms, partial-profile keys, outputs).  Requests are
+ * HBM budget for resident request state (80 bytes per point with FP64
+ * accumulation, the default, 64 with FP32 accumulation: series, statistics,
+ * recurrence-term record, partial-profile keys, outputs).  Requests are
  * admitted in arrival order while they fit; each engine tick launches the
  * FP32-reseeded tile kernel once over a share of the remaining tiles of every
  * admitted request (about tick_cells cells per tick; <= 0 picks 4e9), so
  * requests that arrive while others run join the next launch.  A request's
  * profile/index (P = n-m+1 entries each, host memory the caller keeps alive
  * until completion) are written when it completes.  opts: exclusion, band_k,
- * tile_len (re-seed interval), refine, index_init as for mp_selfjoin_f32r.
+ * tile_len (re-seed interval), refine, index_init, accum64 as for
+ * mp_selfjoin_f32r.
  * submit returns MP_ERR_ADMISSION when a request can never fit the budget.
  * poll/wait return the request's own status and, once done, its latency
  * (submit -> results written) and forget it.  No counterpart in the
