@@ -29,11 +29,14 @@
 
 namespace mp {
 
-// One lane scans the whole series in the reference's sequential order: the
-// dependent FP64 add chain is the critical path.  The wave stages the series
-// through LDS in chunks with coalesced vector loads issued one chunk ahead
-// (vector loads retire in order, so the scan never waits on the chunk in
-// flight); lane 0 reads the staged chunk and writes both prefix arrays.
+// The prefix sums of A and of A^2 are two independent sequential chains (the
+// reference's order, an add chain and an FMA chain).  Each runs in its own
+// workgroup (blockIdx.x 0: A, 1: A^2), where one lane scans the whole series:
+// the dependent FP64 chain and the lane's one store per element are the
+// critical path, so splitting the chains halves both per lane.  The wave
+// stages the series through LDS in chunks with coalesced vector loads issued
+// one chunk ahead (vector loads retire in order, so the scan never waits on
+// the chunk in flight).
 constexpr int kScanChunk = 2048;
 
 __global__ __launch_bounds__(64) void k_ref_cumsum(const double* __restrict__ A, int n,
@@ -42,6 +45,8 @@ __global__ __launch_bounds__(64) void k_ref_cumsum(const double* __restrict__ A,
     __shared__ double buf[2][kScanChunk];
     constexpr int R = kScanChunk / 64;
     const int lane = threadIdx.x;
+    const bool squares = blockIdx.x == 1;
+    double* __restrict__ out = squares ? sq : cs;
     if (n <= 0) return;
     const int nchunks = (n + kScanChunk - 1) / kScanChunk;
     double reg[R];
@@ -60,7 +65,7 @@ __global__ __launch_bounds__(64) void k_ref_cumsum(const double* __restrict__ A,
     load(0);
     stage(0);
     __syncthreads();
-    double c = 0.0, s = 0.0;
+    double c = 0.0;
     for (int ch = 0; ch < nchunks; ++ch) {
         if (ch + 1 < nchunks) load(ch + 1);
         if (lane == 0) {
@@ -69,19 +74,23 @@ __global__ __launch_bounds__(64) void k_ref_cumsum(const double* __restrict__ A,
             const double* b = buf[ch & 1];
             int t = 0;
             if (ch == 0) {
-                c = b[0];
-                s = b[0] * b[0];
-                cs[0] = c;
-                sq[0] = s;
+                c = squares ? b[0] * b[0] : b[0];
+                out[0] = c;
                 t = 1;
             }
+            if (squares) {
 #pragma unroll 16
-            for (; t < cnt; ++t) {
-                const double a = b[t];
-                c = a + c;
-                s = fma(a, a, s);
-                cs[base + t] = c;
-                sq[base + t] = s;
+                for (; t < cnt; ++t) {
+                    const double a = b[t];
+                    c = fma(a, a, c);
+                    out[base + t] = c;
+                }
+            } else {
+#pragma unroll 16
+                for (; t < cnt; ++t) {
+                    c = b[t] + c;
+                    out[base + t] = c;
+                }
             }
         }
         if (ch + 1 < nchunks) stage((ch + 1) & 1);
@@ -639,7 +648,7 @@ namespace mp {
 int launch_ref_stats(const double* A, int n, int m, double* cs, double* sq, double* sInv,
                      double* mus, hipStream_t st) {
     const int P = n - m + 1;
-    hipLaunchKernelGGL(k_ref_cumsum, dim3(1), dim3(64), 0, st, A, n, cs, sq);  // one wave
+    hipLaunchKernelGGL(k_ref_cumsum, dim3(2), dim3(64), 0, st, A, n, cs, sq);  // one wave per chain
     hipLaunchKernelGGL(k_ref_stats, dim3((P + 255) / 256), dim3(256), 0, st, cs, sq, P, m, sInv,
                        mus);
     return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
