@@ -138,9 +138,10 @@ int mp_eval_diag_block_triangle_f64(double prof_colmin[], int64_t idx_colmin[],
  * each series is windows[s]/4 unless opts->exclusion >= 0.  With use_graph
  * the whole batch (statistics, tiles, unpack) is captured once in a hipGraph
  * and replayed while the pointers stay the same.  mp_batch_tune times band
- * widths 4/8 and tile lengths 256..max_tile_len on the real batch, keeps the
- * fastest and records it in an on-disk cache that later mp_batch_create calls
- * consult ($SCRIMP_HIP_TUNE_CACHE, default ~/.cache/scrimp_hip/).
+ * widths 4/8 and tile lengths 256..max_tile_len (<= 0: 8192) on the real
+ * batch, keeps the fastest and records it in an on-disk cache that later
+ * mp_batch_create calls consult ($SCRIMP_HIP_TUNE_CACHE, default
+ * ~/.cache/scrimp_hip/).
  * No counterpart in the reference, which joins one series per run.        */
 typedef struct mp_batch mp_batch;
 int mp_batch_create(mp_ctx* ctx, int32_t nseries, const int64_t* lengths,

@@ -387,7 +387,7 @@ int mp_batch_run(mp_batch* b, const double* d_ts, double* d_profile, int64_t* d_
 int mp_batch_tune(mp_batch* b, const double* d_ts, double* d_profile, int64_t* d_index, int32_t max_tile_len,
                   mp_timing* best) {
     if (!b || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
-    if (max_tile_len <= 0) max_tile_len = 4096;
+    if (max_tile_len <= 0) max_tile_len = 8192;
     const int bks[] = {4, 8};
     const int tls[] = {256, 512, 1024, 2048, 4096, 8192};
     double best_rate = -1.0;
