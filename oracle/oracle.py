@@ -40,6 +40,7 @@ def lib() -> C.CDLL:
         h.orc_pair_dist2.restype = C.c_double
         h.orc_window_moments_ld.argtypes = [_D, C.c_long, C.c_int, _D, _D]
         h.orc_set_fault.argtypes = [C.c_int]
+        h.orc_prescrimp.argtypes = [_D, C.c_long, C.c_int, C.c_double, _L, C.c_long, _D, _L]
         _lib = h
     return _lib
 
@@ -110,6 +111,21 @@ def brute_force(A, m, excl=-1):
     prof = np.empty(P)
     idx = np.empty(P, dtype=np.int64)
     lib().orc_brute_force(_d(A), A.size, m, excl, _d(prof), _l(idx))
+    return prof, idx
+
+
+def prescrimp(A, m, stride=0.25, order=None):
+    """PreSCRIMP phase of scrimppp_orig: (d2 profile, index); order = sampled rows in
+    processing order (ascending multiples of floor(stride*m) by default)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    P = A.size - m + 1
+    prof = np.empty(P)
+    idx = np.empty(P, dtype=np.int64)
+    if order is None:
+        lib().orc_prescrimp(_d(A), A.size, m, float(stride), None, 0, _d(prof), _l(idx))
+    else:
+        o = np.ascontiguousarray(order, dtype=np.int64)
+        lib().orc_prescrimp(_d(A), A.size, m, float(stride), _l(o), o.size, _d(prof), _l(idx))
     return prof, idx
 
 

@@ -427,3 +427,96 @@ void orc_brute_force(const double* A, long n, int m, long excl, double* profile,
     free(cen);
     free(nrm);
 }
+
+/* ScrimpppOrig.cpp:79-283, PreSCRIMP phase (arithmetic in scrimp_oracle.h) */
+void orc_prescrimp(const double* A, long n, int m, double stride, const long* order, long norder,
+                   double* profile, long* index) {
+    const long P = n - m + 1;
+    if (P <= 0) return;
+    const long excl = m / 4;
+    long step = (long)floor(stride * (double)m);
+    if (step < 1) step = 1;
+    double* cs = (double*)malloc(sizeof(double) * n);
+    double* sq = (double*)malloc(sizeof(double) * n);
+    double* mean = (double*)malloc(sizeof(double) * P);
+    double* sig = (double*)malloc(sizeof(double) * P);
+    /* ScrimpppOrig.cpp:84-113 */
+    cs[0] = A[0];
+    sq[0] = A[0] * A[0];
+    for (long i = 1; i < n; ++i) {
+        cs[i] = A[i] + cs[i - 1];
+        sq[i] = fma(A[i], A[i], sq[i - 1]);
+    }
+    for (long k = 0; k < P; ++k) {
+        const double sum = (k == 0) ? cs[m - 1] : cs[m - 1 + k] - cs[k - 1];
+        const double sumsq = (k == 0) ? sq[m - 1] : sq[m - 1 + k] - sq[k - 1];
+        mean[k] = sum / m;
+        sig[k] = sqrt(fma(-mean[k], mean[k], sumsq / m));
+    }
+    for (long i = 0; i < P; ++i) {
+        profile[i] = INFINITY;
+        index[i] = 0;
+    }
+    const double dm = (double)m;
+    long nrows = (P + step - 1) / step;
+    for (long r = 0; r < (order ? norder : nrows); ++r) {
+        const long i = order ? order[r] : r * step;
+        double best = INFINITY;
+        long bestj = 0;
+        for (long j = 0; j < P; ++j) {
+            double d;
+            if (j > i - excl && j < i + excl) {
+                d = INFINITY;
+            } else {
+                double qt = 0.0;
+                for (int t = 0; t < m; ++t) qt = fma(A[j + t], A[i + t], qt);
+                const double mp = (g_fault == 2) ? (dm * mean[j]) : -(dm * mean[j]);
+                d = 2.0 * (dm - fma(mp, mean[i], qt) / (sig[j] * sig[i]));
+            }
+            if (d < best) {
+                best = d;
+                bestj = j;
+            }
+            if (d < profile[j]) {
+                profile[j] = d;
+                index[j] = i;
+            }
+        }
+        profile[i] = best;
+        index[i] = bestj;
+        if (!(best < INFINITY)) continue;
+        const long j = bestj;
+        const double z0 = fma(dm - profile[i] / 2.0, sig[j] * sig[i], (dm * mean[j]) * mean[i]);
+        double z = z0;
+        for (long k = 1; k < step && i + k < P && j + k < P; ++k) {
+            z = (g_fault == 1) ? fma(A[i + k + m - 1], A[j + k + m - 1], z)
+                               : fma(A[i + k + m - 1], A[j + k + m - 1], fma(-A[i + k - 1], A[j + k - 1], z));
+            const double d = 2.0 * (dm - fma(-(dm * mean[j + k]), mean[i + k], z) / (sig[j + k] * sig[i + k]));
+            if (d < profile[i + k]) {
+                profile[i + k] = d;
+                index[i + k] = j + k;
+            }
+            if (d < profile[j + k]) {
+                profile[j + k] = d;
+                index[j + k] = i + k;
+            }
+        }
+        z = z0;
+        for (long k = 1; k < step && i - k >= 0 && j - k >= 0; ++k) {
+            z = fma(A[i - k], A[j - k], fma(-A[i - k + m], A[j - k + m], z));
+            const double d = 2.0 * (dm - fma(-(dm * mean[j - k]), mean[i - k], z) / (sig[j - k] * sig[i - k]));
+            if (d < profile[i - k]) {
+                profile[i - k] = d;
+                index[i - k] = j - k;
+            }
+            if (d < profile[j - k]) {
+                profile[j - k] = d;
+                index[j - k] = i - k;
+            }
+        }
+    }
+    free(cs);
+    free(sq);
+    free(mean);
+    free(sig);
+}

@@ -87,6 +87,33 @@ double orc_pair_dist2(const double* A, long n, int m, long i, long j);
 void orc_window_moments_ld(const double* A, long n, int m, double* mean, double* sigma);
 
 /*
+ * PreSCRIMP, the anytime phase of scrimppp_orig (ScrimpppOrig.cpp:79-203 and
+ * 130-283).  Statistics in that file's own form (ScrimpppOrig.cpp:84-113):
+ *   cs[i] = A[i] + cs[i-1];  sq[i] = fma(A[i], A[i], sq[i-1])
+ *   sum = cs[m-1+k] - cs[k-1];  mean = sum/m;  var = fma(-mean, mean, sumsq/m);  sigma = sqrt(var)
+ * step = floor(stride*m), excl = m/4.  For each sampled row i (0, step, 2*step, ...
+ * in the order given, ascending when `order` is NULL):
+ *   QT(i,j) = sum_t A[j+t]*A[i+t] for every window j -- by FFT in the reference
+ *             (ScrimpppOrig.cpp:137-186), a direct sequential fma sum here
+ *   d(i,j)  = 2*(m - fma(-(m*mean_j), mean_i, QT) / (sigma_j*sigma_i)),
+ *             +inf for i-excl < j < i+excl (ScrimpppOrig.cpp:191-200)
+ *   profile[j] = d(i,j) where smaller (strict <, ascending j), then profile[i] =
+ *   the row minimum and index[i] its first j (ScrimpppOrig.cpp:202-216); then
+ *   the diagonal through (i, j = index[i]) is walked step-1 cells forward and
+ *   back with lastz = fma(m - profile[i]/2, sigma_j*sigma_i, (m*mean_j)*mean_i)
+ *   and lastz = fma(A[i+k+m-1], A[j+k+m-1], fma(-A[i+k-1], A[j+k-1], lastz))
+ *   (backward: fma(A[i-k], A[j-k], fma(-A[i-k+m], A[j-k+m], lastz))), updating
+ *   both profile[i+k] and profile[j+k] where smaller (ScrimpppOrig.cpp:218-281).
+ * A row whose every window is excluded keeps +inf and is not walked (the
+ * reference reads an unset index there).  profile[] holds d2 (+inf where never
+ * reached), index[] starts at 0.  Only ties depend on the order of rows: every
+ * update is a minimum, and the row minimum overwriting profile[i] can never
+ * raise it (d(i,j) = d(j,i)).
+ */
+void orc_prescrimp(const double* A, long n, int m, double stride, const long* order, long norder,
+                   double* profile, long* index);
+
+/*
  * Planted-mistake switch for the oracle's own pinning tests: 0 = none,
  * 1 = drop the A[i]*A[j] term of the recurrence, 2 = flip the sign of the
  * mean product in the score, 3 = report the row index for column results,
