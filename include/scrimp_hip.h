@@ -97,6 +97,24 @@ int mp_selfjoin_f64_dev(mp_ctx* ctx, const double* d_ts, int64_t n, int32_t m,
                         double* d_profile, int64_t* d_index, const mp_opts* opts,
                         mp_timing* timing);
 
+/* ---- PreSCRIMP, the anytime phase of scrimppp_orig --------------------------
+ * Replaces the PreSCRIMP loop of ScrimpppOrig::compute_matrix_profile
+ * (scrimppp/src/ScrimpppOrig.cpp:130-283): every sampled row i = 0, step,
+ * 2*step, ... (step = floor(stride*m), at least 1) gets its full distance
+ * profile, which lowers profile[j] for every j and sets profile[i] to the
+ * row's minimum; the diagonal through (i, that minimum's partner) is then
+ * walked step-1 cells forward and back.  Partners with |i-j| < m/4 are
+ * excluded (ScrimpppOrig.cpp:191-200).  profile[] gets d^2 (+inf where no
+ * update reached), index[] the partner (0 where none, as the reference
+ * initialises it).  The reference's shuffled order of rows only decides
+ * ties, and its FFT-based dot products agree with the direct FP64 sums here
+ * to rounding.  Host pointers; ts has n elements, profile/index P each.     */
+int mp_prescrimp_f64(const double* ts, int64_t n, int32_t m, double stride, double* profile,
+                     int64_t* index, mp_timing* timing);
+/* Same on device pointers. */
+int mp_prescrimp_f64_dev(mp_ctx* ctx, const double* d_ts, int64_t n, int32_t m, double stride,
+                         double* d_profile, int64_t* d_index, mp_timing* timing);
+
 /* ---- self-join, FP32 recurrence with FP64 statistics and re-seeding -------
  * Same result convention.  The covariance recurrence runs in FP32; window
  * statistics use compensated FP64 sums; every tile of tile_len columns is

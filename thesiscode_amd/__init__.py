@@ -17,7 +17,7 @@ import numpy as np
 from ._ffi import Opts, ScrimpError, Timing, check, lib
 
 __all__ = [
-    "ScrimpError", "Device", "selfjoin_f64", "selfjoin_f32r", "eval_diag_block_triangle_f64",
+    "ScrimpError", "Device", "selfjoin_f64", "selfjoin_f32r", "prescrimp_f64", "eval_diag_block_triangle_f64",
     "random_walk", "normals", "uniforms", "precision_series", "profile_length", "exclusion_zone",
     "cells", "ALGORITHMS", "make_opts", "Batch", "Server",
 ]
@@ -84,6 +84,21 @@ def selfjoin_f32r(ts, m: int, *, exclusion: int = -1, band_k: int = 0, tile_len:
     1e-4 of FP64; accum64=False accumulates in FP32 (faster, ~1e-3 on long near-periodic series).
     """
     return _selfjoin("mp_selfjoin_f32r", ts, m, make_opts(exclusion, band_k, tile_len, refine, index_init, accum64))
+
+
+def prescrimp_f64(ts, m: int, stride: float = 0.25):
+    """PreSCRIMP, the anytime phase of scrimppp_orig (ScrimpppOrig.cpp:130-283): an upper
+    bound of the profile, exact at the sampled rows 0, step, 2*step, ... (step = floor(stride*m)).
+    Returns (d2 profile, index, timing); d2 = +inf and index 0 where no update reached."""
+    ts = np.ascontiguousarray(ts, dtype=np.float64)
+    n = ts.shape[0]
+    P = max(0, n - m + 1)
+    prof = np.empty(P, dtype=np.float64)
+    idx = np.empty(P, dtype=np.int64)
+    tm = Timing()
+    rc = lib().mp_prescrimp_f64(_dptr(ts), n, m, float(stride), _dptr(prof), _lptr(idx), C.byref(tm))
+    check(rc, "mp_prescrimp_f64")
+    return prof, idx, tm.as_dict()
 
 
 def eval_diag_block_triangle_f64(prof_colmin, idx_colmin, prof_rowmin, idx_rowmin, tmpQ, blocklen,

@@ -28,7 +28,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["exact_f64.hip", "reseeded_f32.hip", "api.cpp", "batch.cpp", "datagen.cpp", "serve.cpp"]
+SOURCES = ["exact_f64.hip", "reseeded_f32.hip", "prescrimp.hip", "api.cpp", "batch.cpp", "datagen.cpp",
+           "serve.cpp"]
 
 
 def _hipcc() -> str:

@@ -7,12 +7,14 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <new>
 #include <vector>
 
 #include "ctx.hpp"
 #include "exact_f64.hpp"
+#include "prescrimp.hpp"
 #include "reseeded_f32.hpp"
 #include "scrimp_hip.h"
 
@@ -364,6 +366,101 @@ static int host_selfjoin(DevSelfJoin fn, const double* ts, int64_t n, int32_t m,
 int mp_selfjoin_f64(const double* ts, int64_t n, int32_t m, double* profile, int64_t* index,
                     const mp_opts* o, mp_timing* tm) {
     return host_selfjoin(mp_selfjoin_f64_dev, ts, n, m, profile, index, o, tm);
+}
+
+// ---------------------------------------------------------------------------
+// PreSCRIMP (the anytime phase of scrimppp_orig)
+// ---------------------------------------------------------------------------
+int mp_prescrimp_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double stride,
+                         double* d_profile, int64_t* d_index, mp_timing* tm) {
+    if (!c || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
+    if (!(stride > 0.0) || !(stride < 1e9)) return MP_ERR_BAD_ARG;
+    Geometry g;
+    MP_TRY(geometry(n, m, nullptr, &g));
+    const int64_t P = g.P;
+    // the reference's stepSize = floor(stride*m) (ScrimpppOrig.cpp:51), at least 1
+    const int64_t step = std::max<int64_t>(1, static_cast<int64_t>(std::floor(stride * m)));
+    const int64_t nrows = (P + step - 1) / step;
+    std::lock_guard<std::mutex> guard(c->lock);
+    MP_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n, 8) + 2 * aligned_bytes(P + 64, 8) +
+                        2 * aligned_bytes(P, 8);
+    MP_TRY(c->work.reserve(need));
+    c->work.used = 0;
+    double* A = c->work.take<double>(n + 64);
+    double* cs = c->work.take<double>(n);
+    double* sq = c->work.take<double>(n);
+    double* sInv = c->work.take<double>(P + 64);
+    double* mus = c->work.take<double>(P + 64);
+    auto* val = c->work.take<unsigned long long>(P);
+    auto* key = c->work.take<unsigned long long>(P);
+    const int ib = bits_for(P + 1);
+    const unsigned long long v0 = mp::ord_f64_host(-std::numeric_limits<double>::infinity());
+    MP_HIP(hipEventRecord(c->ev[0], st));
+    MP_HIP(hipMemcpyAsync(A, d_ts, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    MP_TRY(mp::launch_fill_u64(reinterpret_cast<unsigned long long*>(A + n), 64, 0ull, st));
+    MP_TRY(mp::launch_ref_stats(A, static_cast<int>(n), m, cs, sq, sInv, mus, st));
+    MP_TRY(mp::launch_fill_u64(val, P, v0, st));
+    MP_TRY(mp::launch_fill_u64(key, P, v0 | ((1ull << ib) - 1ull), st));
+    MP_HIP(hipEventRecord(c->ev[1], st));
+    mp::PreScrimpLaunch a{};
+    a.A = A;
+    a.sInv = sInv;
+    a.mus = mus;
+    a.P = static_cast<int>(P);
+    a.m = m;
+    a.step = static_cast<int>(std::min<int64_t>(step, P));
+    a.nrows = static_cast<int>(nrows);
+    a.excl = m / 4;
+    a.ib = ib;
+    a.cols_per_wave = 1024;
+    a.val = val;
+    a.key = key;
+    MP_TRY(mp::launch_prescrimp(a, st));
+    MP_HIP(hipEventRecord(c->ev[3], st));
+    // untouched entries keep d2 = +inf and index 0, as the reference initialises them
+    MP_TRY(mp::launch_unpack_f64(val, key, static_cast<int>(P), m, ib, 0, d_profile, d_index, st));
+    MP_HIP(hipEventRecord(c->ev[4], st));
+    MP_HIP(hipEventSynchronize(c->ev[4]));
+    if (tm) {
+        std::memset(tm, 0, sizeof(*tm));
+        tm->stats_ms = elapsed(c->ev[0], c->ev[1]);
+        tm->kernel_ms = elapsed(c->ev[1], c->ev[3]);
+        tm->finalize_ms = elapsed(c->ev[3], c->ev[4]);
+        tm->total_ms = elapsed(c->ev[0], c->ev[4]);
+        tm->cells = static_cast<double>(nrows) * static_cast<double>(P);
+        tm->work_items = nrows;
+    }
+    return MP_OK;
+}
+
+int mp_prescrimp_f64(const double* ts, int64_t n, int32_t m, double stride, double* profile,
+                     int64_t* index, mp_timing* tm) {
+    if (!ts || !profile || !index) return MP_ERR_NULL;
+    Geometry g;
+    MP_TRY(geometry(n, m, nullptr, &g));
+    mp_ctx* c = nullptr;
+    MP_TRY(default_ctx(&c));
+    double* dts;
+    double* dprof;
+    int64_t* didx;
+    {
+        std::lock_guard<std::mutex> guard(c->lock);
+        MP_HIP(hipSetDevice(c->device));
+        MP_TRY(c->io.reserve(aligned_bytes(n, 8) + 2 * aligned_bytes(g.P, 8)));
+        c->io.used = 0;
+        dts = c->io.take<double>(n);
+        dprof = c->io.take<double>(g.P);
+        didx = c->io.take<int64_t>(g.P);
+        MP_HIP(hipMemcpyAsync(dts, ts, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    MP_TRY(mp_prescrimp_f64_dev(c, dts, n, m, stride, dprof, didx, tm));
+    std::lock_guard<std::mutex> guard(c->lock);
+    MP_HIP(hipMemcpyAsync(profile, dprof, sizeof(double) * g.P, hipMemcpyDeviceToHost, c->stream));
+    MP_HIP(hipMemcpyAsync(index, didx, sizeof(int64_t) * g.P, hipMemcpyDeviceToHost, c->stream));
+    MP_HIP(hipStreamSynchronize(c->stream));
+    return MP_OK;
 }
 
 // ---------------------------------------------------------------------------
