@@ -107,3 +107,28 @@ def test_cli_self_join_writes_reference_format(tmp_path, golden, test_series, fm
     else:
         vals = np.loadtxt(out)
         np.testing.assert_allclose(vals[:, 0], np.sqrt(np.abs(p_ref)), rtol=1e-15)
+
+
+@pytest.mark.gpu
+def test_cli_orig_writes_prescrimp_and_final_profiles(tmp_path):
+    # scrimppp_orig's drop-in: ASCII in, ASCII out (<out>_prescrimp, <out>);
+    # step = floor(prescrimp_stride * m) with the integer stride parameter
+    import thesiscode_amd as tc
+    from oracle import oracle as orc
+
+    A = tc.random_walk(17, 1500)
+    m, stride = 40, 1
+    src = tmp_path / "series.ascii"  # the command line resolves the type from the suffix
+    src.write_text("\n".join(repr(float(v)) for v in A) + "\n")
+    r = run("scrimppp_hip", "scrimp_hip_orig", m, stride, src, "--output-file", tmp_path / "mp", cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    pre = np.loadtxt(tmp_path / "mp_prescrimp")
+    fin = np.loadtxt(tmp_path / "mp")
+    q, j = orc.prescrimp(A, m, float(stride))
+    np.testing.assert_allclose(pre[:, 0] ** 2, q, rtol=1e-9, atol=1e-9 * 2 * m)
+    assert np.mean(pre[:, 1] == j) >= 0.999
+    # the final profile: the minimum of PreSCRIMP's and the full join's
+    p64, i64, _ = tc.selfjoin_f64(A, m)
+    want = np.minimum(q, p64)
+    np.testing.assert_allclose(fin[:, 0] ** 2, want, rtol=1e-9, atol=1e-9 * 2 * m)
+    assert np.all(fin[:, 0] ** 2 <= pre[:, 0] ** 2 * (1 + 1e-12) + 1e-12)

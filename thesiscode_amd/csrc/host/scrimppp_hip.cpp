@@ -30,7 +30,7 @@ void usage() {
                  "  --algo NAME            algorithm: "
               << MatProfImplFactory::getInstance().get_available_implementations()
               << "\n  --window-length M      size m of the matching window\n"
-                 "  --prescrimp-stride S   accepted for compatibility (no PreSCRIMP stage)\n"
+                 "  --prescrimp-stride S   PreSCRIMP step floor(S*M) of scrimp_hip_orig (other algorithms ignore it)\n"
                  "  --input-file FILE      time series (.bin format 2 or .ascii)\n"
                  "  --output-file FILE     result (default <algo>_new_MatrixProfile_<m><file>.matprof.<type>)\n"
                  "  --blocklen B           accepted for compatibility (bands are sized on the device)\n"
