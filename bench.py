@@ -29,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
+FP64_MATRIX_PEAK_TFLOPS = 78.6  # dense FP64 MFMA
 FLOPS_PER_CELL = 8
 
 WORKLOADS = {
@@ -232,6 +233,58 @@ def run_precision(workload: str, tiles=(256, 1024, 4096, 8192)):
     return {"workload": workload, "n": n, "m": m, "f64_device_ms": t64["total_ms"], "variants": rows}
 
 
+def run_prescrimp(workload: str, stride: float, steps: int, warmup: int, cpu: bool):
+    """PreSCRIMP (scrimppp_orig's anytime phase) on the workload's series: sampled rows'
+    distance profiles on the FP64 matrix cores plus the diagonal walks.  The unit is one
+    distance evaluation of a sampled row against a window (2m flops on the matrix cores).
+    The CPU baseline is the oracle's sequential restatement on a bounded set of rows."""
+    import thesiscode_amd as tc
+
+    kind, seed, n, m, _ = WORKLOADS[workload]
+    A = make_series(kind, seed, n, m)
+    for _ in range(warmup):
+        tc.prescrimp_f64(A, m, stride)
+    t0 = time.perf_counter()
+    tms = []
+    for _ in range(steps):
+        _, _, tm = tc.prescrimp_f64(A, m, stride)
+        tms.append(tm)
+    wall = time.perf_counter() - t0
+    evals = tms[0]["cells"]
+    kernel_ms = min(t["kernel_ms"] for t in tms)
+    achieved = 2.0 * m * evals / (kernel_ms / 1e3) / 1e12
+    out = {"metric": "PreSCRIMP distance evaluations/s", "value": evals * steps / wall, "unit": "evaluations/s",
+           "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * wall / steps,
+           "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic (project Philox4x32-10 series; host buffers, copies included)",
+           "config": {"workload": workload, "n": n, "m": m, "stride": stride,
+                      "step": max(1, int(np.floor(stride * m))), "sampled_rows": tms[0]["work_items"],
+                      "evaluations": evals},
+           "roofline": {"bound": "mfma", "kernel": "k_pre_rows + k_pre_walk", "achieved": achieved,
+                        "peak": FP64_MATRIX_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_MATRIX_PEAK_TFLOPS, "kernel_ms": kernel_ms,
+                        "flops_per_evaluation": 2 * m}}
+    if cpu:
+        from oracle import oracle as orc
+
+        P = n - m + 1
+        step = max(1, int(np.floor(stride * m)))
+        rows = np.arange(0, P, step)
+        k = 1
+        while True:
+            sample = rows[:: max(1, rows.size // k)][:k]
+            t0 = time.perf_counter()
+            orc.prescrimp(A, m, stride, order=sample)
+            dt = time.perf_counter() - t0
+            if dt >= 5.0 or k >= rows.size:
+                break
+            k = min(rows.size, max(k * 2, int(k * 6.0 / max(dt, 1e-3))))
+        out["cpu_baseline"] = {"value": sample.size * P / dt, "unit": "evaluations/s", "cores": 1, "kind": "port",
+                               "sample": f"{workload}: {sample.size} of {rows.size} sampled rows with their walks "
+                                         f"({sample.size * P:.3e} evaluations, {dt:.1f} s)"}
+    return out
+
+
 def copy_probe(gib: int = 1, reps: int = 3):
     """A plain device-to-device copy of known size, to scale rocprof's FETCH_SIZE and
     WRITE_SIZE counters into bytes (run this under `rocprofv3 --pmc ...`)."""
@@ -303,6 +356,9 @@ def main() -> int:
     ap.add_argument("--no-variant", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 1 GiB device to device 3 times (calibration of rocprof byte counters)")
+    ap.add_argument("--prescrimp", action="store_true",
+                    help="run PreSCRIMP (scrimppp_orig's anytime phase) on --workload instead")
+    ap.add_argument("--prescrimp-stride", type=float, default=0.25)
     ap.add_argument("--precision", action="store_true",
                     help="report the FP32-reseeded profile error against FP64 on --workload")
     args = ap.parse_args()
@@ -316,6 +372,10 @@ def main() -> int:
         return 0
     if args.precision:
         print(json.dumps(run_precision(args.workload)), flush=True)
+        return 0
+    if args.prescrimp:
+        print(json.dumps(run_prescrimp(args.workload, args.prescrimp_stride, args.steps, args.warmup,
+                                       not args.no_cpu_baseline)), flush=True)
         return 0
     if args.batch:
         r = run_batch_bench(args.steps, args.warmup, args.tune)

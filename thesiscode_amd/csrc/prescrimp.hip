@@ -70,7 +70,8 @@ __device__ __forceinline__ void merge(unsigned long long* val, unsigned long lon
 }
 
 constexpr int kColsPerChunk = 64;  // 4 MFMA column blocks of 16
-constexpr int kU = 4;              // steps whose loads are issued together
+constexpr int kU = 4;              // steps per group (their loads are issued together)
+static_assert(kU == 4, "the column window rolls by one 16-column sub-block (4 steps) per group");
 
 // One wave: sampled rows 16*rb .. 16*rb+15 against columns [c0, c0 + ncols).
 // MFMA v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16] and
@@ -125,22 +126,31 @@ __global__ __launch_bounds__(256) void k_pre_rows(const double* __restrict__ A,
         v4d acc[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
+        // Column operand of sub-block c at step s is X(s + 4c), X(u) =
+        // A[cb + l16 + 4u + t0]: a window of 16 X values rolls by 4 per group
+        // of 4 steps, so a step needs one new column load instead of four.
+        const int xb = cb + l16 + t0;
+        const int nlast = P + m - 2;  // last element a valid pair reads
+        double xw[16];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) xw[k] = A[min(xb + 4 * k, nlast)];
         for (int s0 = 0; s0 < nsteps_u; s0 += kU) {
-            double a[kU], b[kU][4];
+            double a[4];
 #pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const int t = 4 * (s0 + u) + t0;
-                const int tc = min(t, m - 1);
-                const double x = A[ai + tc];
-                a[u] = (t < m) ? x : 0.0;
-#pragma unroll
-                for (int c = 0; c < 4; ++c) b[u][c] = A[cj[c] + tc];
+            for (int e = 0; e < 4; ++e) {
+                xw[12 + e] = A[min(xb + 4 * (s0 + 12 + e), nlast)];
+                const int t = 4 * (s0 + e) + t0;
+                const double x = A[ai + min(t, m - 1)];
+                a[e] = (t < m) ? x : 0.0;
             }
 #pragma unroll
-            for (int u = 0; u < kU; ++u) {
+            for (int e = 0; e < 4; ++e) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u][c], acc[c], 0, 0, 0);
+                for (int c = 0; c < 4; ++c)
+                    acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[e], xw[e + 4 * c], acc[c], 0, 0, 0);
             }
+#pragma unroll
+            for (int k = 0; k < 12; ++k) xw[k] = xw[k + 4];
         }
         // epilogue: result (r, c) is QT(row 4r + t0, column cb + 16c + l16)
         SP cbest[4];
