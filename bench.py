@@ -363,8 +363,8 @@ def main() -> int:
                     help="report the FP32-reseeded profile error against FP64 on --workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path yet "
+              "(a self-join would shard by band pairs, one rank per GPU); run with --gpus 1", file=sys.stderr)
         return 2
 
     if args.copy_probe:
