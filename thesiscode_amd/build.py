@@ -23,9 +23,15 @@ ARCH = os.environ.get("SCRIMP_HIP_ARCH", "gfx950")
 
 # -ffp-contract=off: the exact path's arithmetic is written with explicit fma()
 # and must not be re-fused.  Packed FP32 instructions are disabled.
+# -amdgpu-mfma-vgpr-form: matrix-core accumulators live in ordinary VGPRs.
+# Without it the compiler keeps them in AGPRs and copies all of them to VGPRs
+# and back around every loop iteration of the MFMA loops (64 moves and a
+# pipeline drain per 16 MFMAs in PreSCRIMP), and AGPRs count against
+# occupancy on top of the VGPRs.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 SOURCES = ["exact_f64.hip", "reseeded_f32.hip", "prescrimp.hip", "api.cpp", "batch.cpp", "datagen.cpp",
