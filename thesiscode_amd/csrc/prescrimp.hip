@@ -7,7 +7,7 @@
 // decides ties: here all rows run at once and merge through the exact path's
 // 64-bit (score, key) atomics, and the walks run after all rows.
 //
-//   k_pre_rows  sliding dot products QT(i, j) of 16 sampled rows against 64
+//   k_pre_rows  sliding dot products QT(i, j) of 32 sampled rows against 64
 //               columns at a time on the FP64 matrix cores, with the score,
 //               the row/column maxima and the merge fused into the epilogue
 //   k_pre_walk  one lane per sampled row: the +-(step-1) diagonal walks
@@ -18,6 +18,8 @@
 // i - m/4 < j < i + m/4 (ScrimpppOrig.cpp:191-200).
 #include "common.hpp"
 #include "prescrimp.hpp"
+
+#include <type_traits>
 
 namespace mp {
 
@@ -69,107 +71,172 @@ __device__ __forceinline__ void merge(unsigned long long* val, unsigned long lon
     atomicMax(&key[e], static_cast<unsigned long long>(make_key(o, ib, partner)));
 }
 
+// buffer resource word 3 for raw (untyped) buffer loads on gfx9-class chips
+constexpr int kBufWord3 = 0x00020000;
 constexpr int kColsPerChunk = 64;  // 4 MFMA column blocks of 16
 constexpr int kU = 4;              // steps per group (their loads are issued together)
 static_assert(kU == 4, "the column window rolls by one 16-column sub-block (4 steps) per group");
 
-// One wave: sampled rows 16*rb .. 16*rb+15 against columns [c0, c0 + ncols).
+// One wave: RB blocks of 16 sampled rows (rows 16*RB*rb ..) against columns
+// [c0, c0 + ncols); the blocks share the column operands.
 // MFMA v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16] and holds D[4r + l / 16][l % 16] in its r-th result; here
 // A[row][k] = A[i_row + 4s + k], B[k][col] = A[j_col + 4s + k], D = QT.
-__global__ __launch_bounds__(256) void k_pre_rows(const double* __restrict__ A,
-                                                  const double* __restrict__ sInv,
-                                                  const double* __restrict__ mus, int P, int m,
-                                                  int step, int nrows, int excl, int ib,
-                                                  int cols_per_wave, int ncolwaves,
-                                                  unsigned long long* val, unsigned long long* key) {
+// Two waves per SIMD (RB = 2 takes 246 VGPRs): one wave's matrix-core work
+// covers the other's epilogue.
+template <int RB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pre_rows(
+    const double* __restrict__ A, const double* __restrict__ sInv, const double* __restrict__ mus,
+    int P, int m, int step, int nrows, int excl, int ib, int cols_per_wave, int ncolwaves,
+    unsigned long long* val, unsigned long long* key) {
     const int lane = threadIdx.x % kWave;
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-    const int rb = static_cast<int>(wave / ncolwaves);
-    const int cw = static_cast<int>(wave % ncolwaves);
-    if (rb * 16 >= nrows) return;
+    // wave-uniform (the buffer resources below must live in scalar registers)
+    const int rb = __builtin_amdgcn_readfirstlane(static_cast<int>(wave / ncolwaves));
+    const int cw = __builtin_amdgcn_readfirstlane(static_cast<int>(wave % ncolwaves));
+    const int row0 = rb * 16 * RB;
+    if (row0 >= nrows) return;
     const int c0 = cw * cols_per_wave;
     const int c1 = min(P, c0 + cols_per_wave);
     const int l16 = lane % 16, t0 = lane / 16;
 
-    // this lane's A-operand row, and the rows 4r + t0 its results belong to
-    const int arow = rb * 16 + l16;
-    const int ai = (arow < nrows) ? arow * step : 0;
-    int ri[4];
-    bool rv[4];
-    double rs[4], rm[4];
+    // Operands come through buffer loads: one resource for the rows' windows
+    // (based at the block's first row), one for the column stream (based at
+    // the wave's first column), a 32-bit per-lane offset and an immediate per
+    // step.  A column read past the end of the series is out of the resource's
+    // range and returns 0; it only feeds columns >= P or padded steps.
+    const int64_t n = static_cast<int64_t>(P) + m - 1;
+    const int64_t rbase = static_cast<int64_t>(row0) * step;
+    auto nrec = [](int64_t elems) __attribute__((always_inline)) {
+        const int64_t b = elems * 8;
+        return b < 0x7FFFFFF0 ? static_cast<int>(b) : 0x7FFFFFF0;
+    };
+    // wave-uniform base addresses and sizes; a resource is built at each load
+    // from them (scalar moves), which keeps it in scalar registers
+    auto uni = [](const double* p) __attribute__((always_inline)) {
+        const uint64_t v = reinterpret_cast<uint64_t>(p);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+        return reinterpret_cast<double*>((static_cast<uint64_t>(hi) << 32) | lo);
+    };
+    double* const prow = uni(A + rbase);
+    double* const pcol = uni(A + c0);
+    const int brow = __builtin_amdgcn_readfirstlane(nrec(n - rbase));
+    const int bcol = __builtin_amdgcn_readfirstlane(nrec(n - c0));
+    auto bload = [](double* p, int bytes, int off) __attribute__((always_inline)) {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(p, 0, bytes, kBufWord3);
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    };
+    // this lane's A-operand byte offset in each block (a row past the last
+    // one reads the block's first row; its results are never merged)
+    int aoff[RB];
+    SP rbest[RB][4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int row = rb * 16 + 4 * r + t0;
-        rv[r] = row < nrows;
-        ri[r] = rv[r] ? row * step : 0;
-        rs[r] = sInv[ri[r]];
-        rm[r] = mus[ri[r]];
+    for (int b = 0; b < RB; ++b) {
+        const int arow = 16 * b + l16;
+        aoff[b] = ((row0 + arow < nrows) ? arow * step + t0 : t0) * 8;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) rbest[b][r] = SP{__builtin_nan(""), 0};
     }
-    SP rbest[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) rbest[r] = SP{__builtin_nan(""), 0};
+    const int xoff = (l16 + t0) * 8;  // X(u) of chunk cb at xoff + 8 (cb - c0 + 4u)
 
     const int nsteps = (m + 3) / 4;
     const int nsteps_u = (nsteps + kU - 1) / kU * kU;  // steps past m read 0 (A-operand)
+    const int G = m / 16;                                // groups of 4 steps inside the window
+    // Column operand of sub-block c at step s is X(s + 4c), X(u) =
+    // A[cb + l16 + 4u + t0]: a window of 16 X values rolls by 4 per group of 4
+    // steps, so a step needs one new column load instead of four.  The loads
+    // of a group are issued one group ahead (the first group of a chunk and
+    // the chunk's first 12 window values before the previous chunk's
+    // epilogue), so the matrix cores do not wait for them.
+    double xw[16];
+    double an[RB][4], xn[4];
+    auto load_group = [&](int cbx, int s0) __attribute__((always_inline)) {
+        const int xo = xoff + 8 * (cbx - c0 + 4 * (s0 + 12));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            xn[e] = bload(pcol, bcol, xo + 32 * e);
+#pragma unroll
+            for (int b = 0; b < RB; ++b) an[b][e] = bload(prow, brow, aoff[b] + 32 * (s0 + e));
+        }
+    };
+    auto load_window = [&](int cbx) __attribute__((always_inline)) {
+        const int xo = xoff + 8 * (cbx - c0);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) xw[k] = bload(pcol, bcol, xo + 32 * k);
+    };
+    load_window(c0);
+    load_group(c0, 0);
     for (int cb = c0; cb < c1; cb += kColsPerChunk) {
-        int cj[4];
+        v4d acc[RB][4];
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[b][c] = v4d{0.0, 0.0, 0.0, 0.0};
+        // MASKED: the group reaches past the window (m % 16 != 0); its
+        // A-operand elements t >= m are zeroed
+        auto group = [&](int s0, auto mask_tag) __attribute__((always_inline)) {
+            double a[RB][4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                xw[12 + e] = xn[e];
+                const bool in = !decltype(mask_tag)::value || 4 * (s0 + e) + t0 < m;
+#pragma unroll
+                for (int b = 0; b < RB; ++b) a[b][e] = in ? an[b][e] : 0.0;
+            }
+            if (s0 + kU < nsteps_u) load_group(cb, s0 + kU);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+#pragma unroll
+                for (int b = 0; b < RB; ++b)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        acc[b][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[b][e], xw[e + 4 * c], acc[b][c], 0, 0, 0);
+            }
+#pragma unroll
+            for (int k = 0; k < 12; ++k) xw[k] = xw[k + 4];
+        };
+        for (int g = 0; g < G; ++g) group(4 * g, std::false_type{});
+        if (4 * G < nsteps_u) group(4 * G, std::true_type{});
+        // the next chunk's first loads, issued before this chunk's epilogue
+        load_window(cb + kColsPerChunk);
+        load_group(cb + kColsPerChunk, 0);
+        // the column statistics, and the column filter: the columns' current
+        // values, read after the products (values only grow, so a later read
+        // filters more)
         double cs[4], cm[4];
         unsigned long long cur[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            cj[c] = min(cb + 16 * c + l16, P - 1);
-            cs[c] = sInv[cj[c]];
-            cm[c] = mus[cj[c]];
-            cur[c] = val[cj[c]] >> ib;  // the column filter, read before the products
+            const int cj = min(cb + 16 * c + l16, P - 1);
+            cs[c] = sInv[cj];
+            cm[c] = mus[cj];
+            cur[c] = val[cj] >> ib;
         }
-        v4d acc[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
-        // Column operand of sub-block c at step s is X(s + 4c), X(u) =
-        // A[cb + l16 + 4u + t0]: a window of 16 X values rolls by 4 per group
-        // of 4 steps, so a step needs one new column load instead of four.
-        const int xb = cb + l16 + t0;
-        const int nlast = P + m - 2;  // last element a valid pair reads
-        double xw[16];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) xw[k] = A[min(xb + 4 * k, nlast)];
-        for (int s0 = 0; s0 < nsteps_u; s0 += kU) {
-            double a[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                xw[12 + e] = A[min(xb + 4 * (s0 + 12 + e), nlast)];
-                const int t = 4 * (s0 + e) + t0;
-                const double x = A[ai + min(t, m - 1)];
-                a[e] = (t < m) ? x : 0.0;
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[e], xw[e + 4 * c], acc[c], 0, 0, 0);
-            }
-#pragma unroll
-            for (int k = 0; k < 12; ++k) xw[k] = xw[k + 4];
-        }
-        // epilogue: result (r, c) is QT(row 4r + t0, column cb + 16c + l16)
+        // epilogue: result (b, r, c) is QT(row 16b + 4r + t0, column cb + 16c + l16)
         SP cbest[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) cbest[c] = SP{__builtin_nan(""), 0};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int b = 0; b < RB; ++b) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int j = cb + 16 * c + l16;
-                const int i = ri[r];
-                const bool ok = rv[r] && j < P && !(j > i - excl && j < i + excl);
-                const double sc = fma(acc[c][r], rs[r] * cs[c], -(rm[r] * cm[c]));
-                const double s = ok ? sc : __builtin_nan("");
-                rbest[r] = pick(rbest[r], SP{s, j});
-                cbest[c] = pick(cbest[c], SP{s, i});
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + 16 * b + 4 * r + t0;
+                const bool rv = row < nrows;
+                const int i = rv ? row * step : 0;
+                const double rs = sInv[i], rm = mus[i];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int j = cb + 16 * c + l16;
+                    const bool ok = rv && j < P && !(j > i - excl && j < i + excl);
+                    const double sc = fma(acc[b][c][r], rs * cs[c], -(rm * cm[c]));
+                    const double s = ok ? sc : __builtin_nan("");
+                    rbest[b][r] = pick(rbest[b][r], SP{s, j});
+                    cbest[c] = pick(cbest[c], SP{s, i});
+                }
             }
         }
-        // columns: the four lane quarters hold the other rows of the block.
+        // columns: the four lane quarters hold the other rows of the blocks.
         // A column result is merged only if its truncated score reaches the
         // truncated current value (values only grow; comparing the truncated
         // scores the keys hold keeps the competing keys independent of timing)
@@ -182,13 +249,17 @@ __global__ __launch_bounds__(256) void k_pre_rows(const double* __restrict__ A,
     }
     // rows: the 16 lanes of a DPP row hold the other columns
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        SP b = rbest[r];
-        b = ror_pick<8>(b);
-        b = ror_pick<4>(b);
-        b = ror_pick<2>(b);
-        b = ror_pick<1>(b);
-        if (l16 == r && rv[r] && b.s == b.s) merge(val, key, ri[r], b.s, b.p, ib);
+    for (int b = 0; b < RB; ++b) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            SP x = rbest[b][r];
+            x = ror_pick<8>(x);
+            x = ror_pick<4>(x);
+            x = ror_pick<2>(x);
+            x = ror_pick<1>(x);
+            const int row = row0 + 16 * b + 4 * r + t0;
+            if (l16 == r && row < nrows && x.s == x.s) merge(val, key, static_cast<int64_t>(row) * step, x.s, x.p, ib);
+        }
     }
 }
 
@@ -232,12 +303,20 @@ __global__ void k_pre_walk(const double* __restrict__ A, const double* __restric
 
 int launch_prescrimp(const PreScrimpLaunch& a, hipStream_t st) {
     if (a.nrows <= 0) return MP_OK;
-    const int rblocks = (a.nrows + 15) / 16;
+    // two blocks of 16 rows per wave: twice the MFMAs per column load (three
+    // no longer fit in the VGPRs)
+    constexpr int RB = 2;
+    const int rblocks = (a.nrows + 16 * RB - 1) / (16 * RB);
     const int ncolwaves = (a.P + a.cols_per_wave - 1) / a.cols_per_wave;
     const int64_t waves = static_cast<int64_t>(rblocks) * ncolwaves;
     const int64_t groups = (waves + 3) / 4;
     if (groups > 0x7FFFFFFF) return MP_ERR_TOO_LONG;
-    hipLaunchKernelGGL(k_pre_rows, dim3(static_cast<unsigned>(groups)), dim3(256), 0, st, a.A, a.sInv,
+    // the kernel's buffer offsets are 32-bit byte offsets from the block's
+    // first row and from the wave's first column
+    const int64_t row_span = (static_cast<int64_t>(16 * RB) * a.step + a.m + 16) * 8;
+    const int64_t col_span = (static_cast<int64_t>(a.cols_per_wave) + kColsPerChunk + a.m + 128) * 8;
+    if (row_span >= 0x7FFFFFF0 || col_span >= 0x7FFFFFF0) return MP_ERR_TOO_LONG;
+    hipLaunchKernelGGL(k_pre_rows<RB>, dim3(static_cast<unsigned>(groups)), dim3(256), 0, st, a.A, a.sInv,
                        a.mus, a.P, a.m, a.step, a.nrows, a.excl, a.ib, a.cols_per_wave, ncolwaves,
                        a.val, a.key);
     if (hipGetLastError() != hipSuccess) return MP_ERR_DEVICE;
