@@ -51,9 +51,11 @@ typedef struct mp_opts {
     int32_t index_init;  /* index reported when nothing beat the initial score: -1 (the
                             parallel drivers) or 0 (scrimp_vert_blocked,
                             ScrimpVertBlocked.cpp:104)                                     */
-    int32_t accum64;     /* FP32 path: 1 (default) = FP32 recurrence terms accumulated in
-                            FP64, within 1e-4 of the FP64 profile; 0 = FP32 accumulation,
-                            about 25% faster, drifts to ~1e-3 on long near-periodic series */
+    int32_t accum64;     /* FP32 path: 1 (default) = each diagonal's covariance kept in
+                            FP64, advanced by FP32 partial sums of 16 steps folded in
+                            every 16 columns; within 1e-4 of the FP64 profile. 0 = FP32
+                            accumulation, about 10% faster, drifts to ~1e-3 on long
+                            near-periodic series                                          */
     int32_t reserved[2];
 } mp_opts;
 
@@ -174,9 +176,8 @@ int mp_batch_destroy(mp_batch* batch);
 
 /* ---- serving: continuous batching of self-join requests --------------------
  * mp_server_create starts an engine thread on the context's device with an
- * HBM budget for resident request state (80 bytes per point with FP64
- * accumulation, the default, 64 with FP32 accumulation: series, statistics,
- * recurrence-term record, partial-profile keys, outputs).  Requests are
+ * HBM budget for resident request state (64 bytes per point: series,
+ * statistics, recurrence-term record, partial-profile keys, outputs).  Requests are
  * admitted in arrival order while they fit; each engine tick launches the
  * FP32-reseeded tile kernel once over a share of the remaining tiles of every
  * admitted request (about tick_cells cells per tick; <= 0 picks 4e9), so

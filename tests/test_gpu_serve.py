@@ -60,7 +60,7 @@ def test_requests_are_within_fp32_tolerance_of_exact():
 
 def test_admission_by_hbm_footprint():
     with tc.Device(0) as dev:
-        # 60 bytes per point: 4 MB holds ~69k points
+        # 64 bytes per point: 4 MB holds ~65k points
         with tc.Server(dev, hbm_budget_bytes=4 << 20) as srv:
             with pytest.raises(tc.ScrimpError) as e:
                 srv.submit(tc.random_walk(3, 200000), 128)

@@ -80,8 +80,9 @@ def selfjoin_f32r(ts, m: int, *, exclusion: int = -1, band_k: int = 0, tile_len:
                   refine: bool = True, index_init: int = -1, accum64: bool = True):
     """FP32 recurrence, FP64 statistics and per-tile FP64 seeds (tile_len = re-seed interval).
 
-    accum64 (default) accumulates the FP32 terms in FP64, which keeps the profile within
-    1e-4 of FP64; accum64=False accumulates in FP32 (faster, ~1e-3 on long near-periodic series).
+    accum64 (default) keeps each diagonal's covariance in FP64, advanced by FP32 partial
+    sums over 16 steps, which keeps the profile within 1e-4 of FP64; accum64=False
+    accumulates in FP32 (faster, ~1e-3 on long near-periodic series).
     """
     return _selfjoin("mp_selfjoin_f32r", ts, m, make_opts(exclusion, band_k, tile_len, refine, index_init, accum64))
 

@@ -520,7 +520,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     MP_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n + 64, 8) +
-                        aligned_bytes(n + 64, mp::term_rec_bytes(accum64)) + aligned_bytes(n + 64, 8) +
+                        aligned_bytes(n + 64, mp::term_rec_bytes()) + aligned_bytes(n + 64, 8) +
                         aligned_bytes(1, sizeof(mp::SeriesDesc)) + 3 * aligned_bytes(2, 8) +
                         aligned_bytes(nbands + 1, 8) + aligned_bytes(1, 4);
     MP_TRY(c->work.reserve(need));
@@ -528,8 +528,7 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     double* A = c->work.take<double>(n + 64);
     double* mu = c->work.take<double>(n + 64);
     double* sigma = c->work.take<double>(n + 64);
-    void* terms = accum64 ? static_cast<void*>(c->work.take<mp::TermRec<double>>(n + 64))
-                          : static_cast<void*>(c->work.take<mp::TermRec<float>>(n + 64));
+    void* terms = c->work.take<mp::TermRec>(n + 64);
     auto* keys = c->work.take<unsigned long long>(n + 64);
     auto* ddesc = c->work.take<mp::SeriesDesc>(1);
     auto* dep = c->work.take<int64_t>(2);

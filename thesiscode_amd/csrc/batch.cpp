@@ -134,7 +134,7 @@ namespace {
 int upload_plan(mp_batch* b) {
     const mp::BatchPlan& p = b->plan;
     const int64_t N = p.total_n + 64;
-    const size_t need = 2 * al(N, 8) + al(N, mp::term_rec_bytes(b->accum64)) + al(N, 8) +
+    const size_t need = 2 * al(N, 8) + al(N, mp::term_rec_bytes()) + al(N, 8) +
                         al(p.nseries, sizeof(mp::SeriesDesc)) +
                         3 * al(p.nseries + 1, 8) + al(p.bprefix.size(), 8) + al(1, 4);
     if (need > b->bytes) {
@@ -153,7 +153,7 @@ int upload_plan(mp_batch* b) {
     mp::F32rBatch& d = b->dev;
     d.mu = reinterpret_cast<double*>(take(al(N, 8)));
     d.sigma = reinterpret_cast<double*>(take(al(N, 8)));
-    d.terms = take(al(N, mp::term_rec_bytes(b->accum64)));
+    d.terms = take(al(N, mp::term_rec_bytes()));
     d.keys = reinterpret_cast<unsigned long long*>(take(al(N, 8)));
     d.accum64 = b->accum64;
     auto* dd = reinterpret_cast<mp::SeriesDesc*>(take(al(p.nseries, sizeof(mp::SeriesDesc))));

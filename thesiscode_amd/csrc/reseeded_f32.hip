@@ -116,7 +116,7 @@ __global__ void k_prep_terms(const SeriesDesc* __restrict__ descs,
                              const int64_t* __restrict__ entry_prefix, int nseries,
                              int64_t nentries, const double* __restrict__ A,
                              const double* __restrict__ mu, const double* __restrict__ sigma,
-                             void* __restrict__ terms, int accum64) {
+                             TermRec* __restrict__ terms) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= nentries) return;
     const int s = find_series(entry_prefix, nseries, e);
@@ -131,13 +131,7 @@ __global__ void k_prep_terms(const SeriesDesc* __restrict__ descs,
         f = static_cast<float>(0.5 * (a[i + d.m] - a[i]));
         g = static_cast<float>((a[i + d.m] - mu[o + 1]) + (a[i] - mu[o]));
     }
-    if (accum64) {
-        // the same FP32-rounded terms, widened: only the accumulation differs
-        static_cast<TermRec<double>*>(terms)[o] =
-            TermRec<double>{static_cast<double>(f), static_cast<double>(g), iv, 0.0f, 0.0};
-    } else {
-        static_cast<TermRec<float>*>(terms)[o] = TermRec<float>{f, g, iv, 0.0f};
-    }
+    terms[o] = TermRec{f, g, iv, 0.0f};
 }
 
 __global__ void k_init_keys(const SeriesDesc* __restrict__ descs,
@@ -184,19 +178,13 @@ __device__ __forceinline__ double key_max_quarters(double x) {
     return pair(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
 }
 
-template <typename Acc>
-__device__ __forceinline__ Acc shift_down_acc(Acc src, Acc fill);
-template <>
-__device__ __forceinline__ float shift_down_acc<float>(float src, float fill) {
-    return shift_down_f32(src, fill);
-}
-template <>
-__device__ __forceinline__ double shift_down_acc<double>(double src, double fill) {
-    return shift_down_f64(src, fill);
-}
-
-// Acc = float: FP32 recurrence with FP32 accumulation.  Acc = double: the same
-// FP32-rounded terms accumulated in FP64.
+// ACC64 = false: FP32 recurrence with FP32 accumulation.  ACC64 = true: each
+// diagonal's covariance is an FP64 sum covd plus an FP32 partial cov of the
+// current flush window's increments, folded into covd when the next window
+// starts; a cell's correlation is taken from fl32(covd) and the partial.  The
+// FP32 partial spans 16 steps, so its rounding stays about m/16 times below
+// that of a covariance accumulated in FP32, while a step costs FP32
+// instructions only (the fold adds three FP64-class ones per 16 cells).
 //
 // Per step the wave needs two uniform records: the column's terms and the
 // terms of the row entering lane 63.  They are scalar loads issued one step
@@ -205,13 +193,13 @@ __device__ __forceinline__ double shift_down_acc<double>(double src, double fill
 // before it issues anything.  The step's column partial and the row leaving
 // lane 0 are written to LDS by the next step, after that wait, so the wait
 // never covers LDS writes issued just before it.
-template <int K, typename Acc>
+template <int K, bool ACC64>
 __device__ __forceinline__ void tiles_body(
     const SeriesDesc* __restrict__ descs, const int64_t* __restrict__ tile_prefix,
     const int64_t* __restrict__ band_prefix, int nseries, const double* __restrict__ A,
-    const double* __restrict__ mu, const TermRec<Acc>* __restrict__ terms,
+    const double* __restrict__ mu, const TermRec* __restrict__ terms,
     unsigned long long* keys, unsigned int* counter, int64_t ntiles, int T) {
-    using Rec = TermRec<Acc>;
+    using Rec = TermRec;
     constexpr int F = kFlush;
     constexpr int BW = kWave * K;
     static_assert(F % K == 0, "flush period must be a multiple of K");
@@ -309,12 +297,20 @@ __device__ __forceinline__ void tiles_body(
         // Row records: recurrence terms and 1/norm.  Rows past the triangle
         // (row >= tl) are clamped to its last row: every cell of such a row lies
         // outside the triangle, and the windows that reach them mask those cells.
-        Acc cov[K], wdf[K], wdg[K];
+        float cov[K], wdf[K], wdg[K];
+        double covd[ACC64 ? K : 1];
+        float chi[ACC64 ? K : 1];  // fl32(covd), refreshed at each fold
         float wiv[K];
         double rk[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            cov[k] = static_cast<Acc>(seeds[lid0 + k]);
+            if constexpr (ACC64) {
+                covd[k] = seeds[lid0 + k];
+                chi[k] = static_cast<float>(covd[k]);
+                cov[k] = 0.0f;
+            } else {
+                cov[k] = static_cast<float>(seeds[lid0 + k]);
+            }
             const Rec t = terms[roff + min(cb + dbase + lid0 + k, tl - 1)];
             wdf[k] = t.df;
             wdg[k] = t.dg;
@@ -393,17 +389,24 @@ __device__ __forceinline__ void tiles_body(
             cur_f = terms[roff + feed_row(j + 1)];
             if constexpr (W > 0) put_partials(W - 1);
             __builtin_amdgcn_sched_barrier(0);
-            const Acc dfh = c.df, dgh = c.dg;
+            const float dfh = c.df, dgh = c.dg;
             const float ivh = c.inv;
             double ck;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int sl = (S + k) % K;
-                float c1 = fmaf(static_cast<float>(cov[k]) * wiv[sl], ivh, 1.0f);
+                // the same three roundings in both forms: of the covariance to
+                // FP32, of its product with 1/norm_i, and of the final FMA
+                float c1;
+                if constexpr (ACC64) {
+                    c1 = fmaf(fmaf(chi[k], wiv[sl], cov[k] * wiv[sl]), ivh, 1.0f);
+                } else {
+                    c1 = fmaf(cov[k] * wiv[sl], ivh, 1.0f);
+                }
                 if constexpr (EDGE) c1 = (lid0 + k < L - j) ? c1 : -1.0f;
                 const double key = __hiloint2double(__float_as_int(c1), lid0 + k);
-                cov[k] = fma(wdf[sl], dgh, cov[k]);
-                cov[k] = fma(dfh, wdg[sl], cov[k]);
+                cov[k] = fmaf(wdf[sl], dgh, cov[k]);
+                cov[k] = fmaf(dfh, wdg[sl], cov[k]);
                 rk[sl] = key_max(rk[sl], key);
                 ck = (k == 0) ? key : key_max(ck, key);
             }
@@ -415,12 +418,20 @@ __device__ __forceinline__ void tiles_body(
                 const int hi = shift_down_i32_zero(static_cast<int>(rb >> 32));
                 rk[S] = __hiloint2double(hi, lo);
             }
-            wdf[S] = shift_down_acc<Acc>(wdf[S], f.df);
-            wdg[S] = shift_down_acc<Acc>(wdg[S], f.dg);
+            wdf[S] = shift_down_f32(wdf[S], f.df);
+            wdg[S] = shift_down_f32(wdg[S], f.dg);
             wiv[S] = shift_down_f32(wiv[S], f.inv);
             return true;
         };
         auto window = [&](int c0, auto edge_tag, auto tail_tag) __attribute__((always_inline)) {
+            if constexpr (ACC64) {  // fold the last window's partials
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    covd[k] += static_cast<double>(cov[k]);
+                    cov[k] = 0.0f;
+                    chi[k] = static_cast<float>(covd[k]);
+                }
+            }
             load_filter(c0);
             bool go = true;
             [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
@@ -462,20 +473,20 @@ __device__ __forceinline__ void tiles_body(
 #define MP_TILE_PARAMS                                                                          \
     const SeriesDesc *__restrict__ descs, const int64_t *__restrict__ tile_prefix,              \
         const int64_t *__restrict__ band_prefix, int nseries, const double *__restrict__ A,     \
-        const double *__restrict__ mu, const TermRec<Acc> *__restrict__ terms,                  \
+        const double *__restrict__ mu, const TermRec *__restrict__ terms,                       \
         unsigned long long *keys, unsigned int *counter, int64_t ntiles, int T
 #define MP_TILE_ARGS descs, tile_prefix, band_prefix, nseries, A, mu, terms, keys, counter, ntiles, T
 
 // K <= 8: at most 128 VGPRs, so that 4 waves share each SIMD.
-template <int K, typename Acc>
+template <int K, bool ACC64>
 __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(4))) void k_tiles_f32(
     MP_TILE_PARAMS) {
-    tiles_body<K, Acc>(MP_TILE_ARGS);
+    tiles_body<K, ACC64>(MP_TILE_ARGS);
 }
 // K = 16 holds twice the row records; capping its registers would spill.
-template <int K, typename Acc>
+template <int K, bool ACC64>
 __global__ __launch_bounds__(kGroup) void k_tiles_f32_wide(MP_TILE_PARAMS) {
-    tiles_body<K, Acc>(MP_TILE_ARGS);
+    tiles_body<K, ACC64>(MP_TILE_ARGS);
 }
 #undef MP_TILE_PARAMS
 #undef MP_TILE_ARGS
@@ -600,7 +611,7 @@ int launch_f32r_prep(const F32rBatch& b, hipStream_t st) {
                            b.descs, b.stat_prefix, b.nseries, b.nstat, b.A, b.mu, b.sigma);
     const unsigned g = static_cast<unsigned>((b.nentries + 255) / 256);
     hipLaunchKernelGGL(k_prep_terms, dim3(g), dim3(256), 0, st, b.descs, b.entry_prefix,
-                       b.nseries, b.nentries, b.A, b.mu, b.sigma, b.terms, b.accum64);
+                       b.nseries, b.nentries, b.A, b.mu, b.sigma, static_cast<TermRec*>(b.terms));
     hipLaunchKernelGGL(k_init_keys, dim3(g), dim3(256), 0, st, b.descs, b.entry_prefix,
                        b.nseries, b.nentries, b.keys);
     return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
@@ -611,14 +622,14 @@ int launch_f32r_tiles(const F32rBatch& b, int groups, hipStream_t st) {
 #define MP_TILE_CASE(KK, KERNEL)                                                                \
     case KK:                                                                                    \
         if (b.accum64)                                                                          \
-            hipLaunchKernelGGL((KERNEL<KK, double>), dim3(groups), dim3(kGroup), 0, st,         \
+            hipLaunchKernelGGL((KERNEL<KK, true>), dim3(groups), dim3(kGroup), 0, st,           \
                                b.descs, b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu,     \
-                               static_cast<const TermRec<double>*>(b.terms), b.keys, b.counter, \
+                               static_cast<const TermRec*>(b.terms), b.keys, b.counter,         \
                                b.ntiles, b.tile_len);                                           \
         else                                                                                    \
-            hipLaunchKernelGGL((KERNEL<KK, float>), dim3(groups), dim3(kGroup), 0, st,          \
+            hipLaunchKernelGGL((KERNEL<KK, false>), dim3(groups), dim3(kGroup), 0, st,          \
                                b.descs, b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu,     \
-                               static_cast<const TermRec<float>*>(b.terms), b.keys, b.counter,  \
+                               static_cast<const TermRec*>(b.terms), b.keys, b.counter,         \
                                b.ntiles, b.tile_len);                                           \
         break;
     switch (b.band_k) {

@@ -35,30 +35,20 @@ struct F32rBatch {
     int band_k, tile_len;
     const double* A;
     double *mu, *sigma;
-    void* terms;   // one TermRec<double> (accum64) or TermRec<float> per entry
-    int accum64;
+    void* terms;   // one TermRec per entry
+    int accum64;   // covariance in FP64 (folded per-window FP32 partials) or FP32
     unsigned long long* keys;
     unsigned int* counter;
 };
 
 // The recurrence terms of window i, read together: df = (A[i+m] - A[i]) / 2,
 // dg = (A[i+m] - mu[i+1]) + (A[i] - mu[i]) (both rounded to FP32; zero for the
-// last window) and inv = 1 / (sigma sqrt(m)) (0 for a flat window).  With FP64
-// accumulation the FP32-rounded terms are stored widened.  The tile kernel
-// loads one record per column and per row fed into its band.
-template <typename Acc>
-struct TermRec;
-template <>
-struct alignas(16) TermRec<float> {
+// last window) and inv = 1 / (sigma sqrt(m)) (0 for a flat window).  The tile
+// kernel loads one record per column and per row fed into its band.
+struct alignas(16) TermRec {
     float df, dg, inv, pad;
 };
-template <>
-struct alignas(32) TermRec<double> {
-    double df, dg;
-    float inv, pad;
-    double pad2;
-};
-inline size_t term_rec_bytes(int accum64) { return accum64 ? sizeof(TermRec<double>) : sizeof(TermRec<float>); }
+inline size_t term_rec_bytes() { return sizeof(TermRec); }
 
 // Tiles of a band of L columns (the band's longest diagonal).  Columns
 // [0, band_main_cols) are cut into main tiles of at most T columns, made of

@@ -2,7 +2,7 @@
 //
 // A request is one FP32-reseeded self-join.  Submissions queue on the host;
 // an engine thread admits them in arrival order while their device footprint
-// (series, statistics, recurrence terms, partial-profile keys, outputs: 60
+// (series, statistics, recurrence terms, partial-profile keys, outputs: 64
 // bytes per point) fits the HBM budget, keeps that state resident until the
 // request completes, and runs "ticks": each tick is one launch of the
 // persistent tile kernel over a slice of the remaining tiles of every active
@@ -570,12 +570,12 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     const double tc = tick_cells > 0 ? static_cast<double>(tick_cells) : 4e9;
     s->tick_tiles = std::max<int64_t>(1, static_cast<int64_t>(tc / tile_cells));
     // device pools: A, mu, sigma, profile, keys, index and a term record per point
-    const int64_t per_elem = 8 * 6 + static_cast<int64_t>(mp::term_rec_bytes(s->accum64));
+    const int64_t per_elem = 8 * 6 + static_cast<int64_t>(mp::term_rec_bytes());
     s->bytes_per_elem = per_elem;
     const int64_t cap = static_cast<int64_t>(hbm_budget_bytes / per_elem);
     if (cap < 1024) return fail(MP_ERR_BAD_ARG);
     const int64_t band_cap = cap / 64 + 1024;
-    const size_t bytes = mp::aligned_bytes(cap, 8) * 6 + mp::aligned_bytes(cap, mp::term_rec_bytes(s->accum64)) +
+    const size_t bytes = mp::aligned_bytes(cap, 8) * 6 + mp::aligned_bytes(cap, mp::term_rec_bytes()) +
                          mp::aligned_bytes(band_cap, 8) + 512;
     if (hipMalloc(&s->dmem, bytes) != hipSuccess) return fail(MP_ERR_OOM);
     char* p = s->dmem;
@@ -590,7 +590,7 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     s->prof = reinterpret_cast<double*>(take(mp::aligned_bytes(cap, 8)));
     s->keys = reinterpret_cast<unsigned long long*>(take(mp::aligned_bytes(cap, 8)));
     s->idx = reinterpret_cast<int64_t*>(take(mp::aligned_bytes(cap, 8)));
-    s->terms = take(mp::aligned_bytes(cap, mp::term_rec_bytes(s->accum64)));
+    s->terms = take(mp::aligned_bytes(cap, mp::term_rec_bytes()));
     s->bandpool = reinterpret_cast<int64_t*>(take(mp::aligned_bytes(band_cap, 8)));
     s->counter = reinterpret_cast<unsigned int*>(take(512));  // two counters, 256 bytes apart
     s->pool.reset(cap);
