@@ -491,8 +491,30 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32_wide(MP_TILE_PARAMS) {
 #undef MP_TILE_PARAMS
 #undef MP_TILE_ARGS
 
-// One wave per profile entry.  d^2 = 2m(1 - rho): rho from the FP32 key, or
-// recomputed in FP64 from the centred dot product of the winning pair.
+// d^2 = 2m(1 - rho): rho from the FP32 key, or recomputed in FP64 from the
+// centred dot product of the winning pair.  One wave per chunk of 64
+// consecutive entries (lane l: entry e0 + l): the series of the chunk's first
+// entry is found once, and a lane whose entry lies in a later series advances
+// from there.  With refine, the dot products run four entries at a time, one
+// 16-lane row per entry (rows reduce with row_ror, so every lane of a row holds
+// the same sum); each lane then finishes its own entry and the wave stores the
+// chunk coalesced.
+constexpr int kUnpackChunk = 64;
+
+__device__ __forceinline__ double row_sum16(double x) {
+    auto ror = [](double v, auto ctl_tag) __attribute__((always_inline)) {
+        constexpr int ctl = decltype(ctl_tag)::value;
+        const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), ctl, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), ctl, 0xF, 0xF, false);
+        return __hiloint2double(hi, lo);
+    };
+    x += ror(x, std::integral_constant<int, 0x128>{});  // row_ror:8
+    x += ror(x, std::integral_constant<int, 0x124>{});  // row_ror:4
+    x += ror(x, std::integral_constant<int, 0x122>{});  // row_ror:2
+    x += ror(x, std::integral_constant<int, 0x121>{});  // row_ror:1
+    return x;
+}
+
 __global__ __launch_bounds__(256) void k_unpack_f32r(
     const SeriesDesc* __restrict__ descs, const int64_t* __restrict__ entry_prefix, int nseries,
     int64_t nentries, const double* __restrict__ A, const double* __restrict__ mu,
@@ -501,16 +523,25 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
     const int lane = threadIdx.x % kWave;
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
-    for (int64_t e = wave; e < nentries; e += nwaves) {
-        const int s = find_series(entry_prefix, nseries, e);
+    const int row = lane / 16, rl = lane % 16;
+    for (int64_t e0 = wave * kUnpackChunk; e0 < nentries; e0 += nwaves * kUnpackChunk) {
+        const int64_t e = e0 + lane;
+        const bool live = e < nentries;
+        int s = find_series(entry_prefix, nseries, e0);
+        if (live) {
+            while (s + 1 < nseries && entry_prefix[s + 1] <= e) ++s;
+        }
         const SeriesDesc d = descs[s];
-        const int i = static_cast<int>(e - entry_prefix[s]);
-        const unsigned long long key = keys[d.off + i];
+        const int i = live ? static_cast<int>(e - entry_prefix[s]) : 0;
+        const int64_t ai = d.off + i;
+        const unsigned long long key = live ? keys[ai] : 0xFFFFFFFFull;
         const uint32_t low = static_cast<uint32_t>(key);
         const double dm = static_cast<double>(d.m);
-        double d2;
-        int64_t idx;
-        if (low == 0xFFFFFFFFu || !(sigma[d.off + i] > 0.0)) {
+        double d2 = 0.0;
+        int64_t idx = 0;
+        int p = -1;  // the partner whose distance is recomputed
+        if (!live) {
+        } else if (low == 0xFFFFFFFFu || !(sigma[ai] > 0.0)) {
             // nothing beat the initial score -1.0, or a flat window (zero
             // variance), whose every score is NaN in FP64 and never wins
             d2 = 2.0 * dm + 2.0;
@@ -519,22 +550,34 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
             d2 = __builtin_nan("");  // a partner outside the profile: reported, never dereferenced
             idx = -2;
         } else {
-            const int p = static_cast<int>(0xFFFFFFFEu - low);
+            p = static_cast<int>(0xFFFFFFFEu - low);
             idx = p;
             const float c1 = __uint_as_float(static_cast<uint32_t>(key >> 32));
             d2 = 2.0 * dm * (2.0 - static_cast<double>(c1));
-            if (refine) {
-                const double* a = A + d.off;
-                const double mi = mu[d.off + i], mp = mu[d.off + p];
+        }
+        if (refine && __builtin_amdgcn_ballot_w64(p >= 0) != 0) {
+            const int need = (p >= 0) ? d.m : 0;
+            const int64_t ap = d.off + (p >= 0 ? p : 0);
+            const double mi = (p >= 0) ? mu[ai] : 0.0, mp = (p >= 0) ? mu[ap] : 0.0;
+            double mine = 0.0;
+            for (int u = 0; u < kUnpackChunk / 4; ++u) {
+                const int q = 4 * u + row;  // the entry this row sums in round u
+                const int qm = __shfl(need, q);
+                const int64_t qi = __shfl(ai, q), qp = __shfl(ap, q);
+                const double qmi = __shfl(mi, q), qmp = __shfl(mp, q);
                 double acc = 0.0;
-                for (int t = lane; t < d.m; t += kWave) acc = fma(a[i + t] - mi, a[p + t] - mp, acc);
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-                const double den = dm * sigma[d.off + i] * sigma[d.off + p];
-                if (den > 0.0) d2 = 2.0 * dm * (1.0 - acc / den);
+                for (int t = rl; t < qm; t += 16) acc = fma(A[qi + t] - qmi, A[qp + t] - qmp, acc);
+                acc = row_sum16(acc);
+                // entry `lane` was summed by row lane % 4 in round lane / 4
+                const double got = __shfl(acc, 16 * (lane % 4));
+                if (lane / 4 == u) mine = got;
+            }
+            if (p >= 0) {
+                const double den = dm * sigma[ai] * sigma[ap];
+                if (den > 0.0) d2 = 2.0 * dm * (1.0 - mine / den);
             }
         }
-        if (lane == 0) {
+        if (live) {
             profile[d.out_off + i] = d2;
             index[d.out_off + i] = idx;
         }
@@ -645,7 +688,8 @@ int launch_f32r_tiles(const F32rBatch& b, int groups, hipStream_t st) {
 
 int launch_f32r_unpack(const F32rBatch& b, int refine, int64_t index_init, double* profile,
                        int64_t* index, hipStream_t st) {
-    const int64_t waves = b.nentries;
+    if (b.nentries <= 0) return MP_OK;
+    const int64_t waves = (b.nentries + kUnpackChunk - 1) / kUnpackChunk;
     const int64_t gw = (waves + 3) / 4;
     const unsigned g = static_cast<unsigned>(gw < 65536 ? gw : 65536);
     hipLaunchKernelGGL(k_unpack_f32r, dim3(g), dim3(256), 0, st, b.descs, b.entry_prefix,
