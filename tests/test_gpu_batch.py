@@ -44,6 +44,21 @@ def test_batch_matches_single_series():
         assert tm["cells"] == sum(tc.cells(int(n), int(m)) for n, m in zip(lengths, windows))
 
 
+def test_batch_of_short_series_matches_single_series():
+    # profiles of 3 to 90 entries: one unpack chunk of 64 entries spans many series
+    u = tc.uniforms(21, 120, stream=7)
+    windows = np.array([16, 24, 32])[np.minimum((u[60:] * 3).astype(int), 2)].astype(np.int32)
+    lengths = (windows + windows // 4 + 2 + np.floor(u[:60] * 88)).astype(np.int64)
+    series = [tc.random_walk(7000 + s, int(n), stream=3) for s, n in enumerate(lengths)]
+    with tc.Device(0) as dev:
+        b, prof, idx, tm = run_batch(dev, lengths, windows, series, False, band_k=4, tile_len=256)
+        oo = b.out_offsets
+        for s in range(len(series)):
+            p1, i1, _ = tc.selfjoin_f32r(series[s], int(windows[s]), band_k=4, tile_len=256)
+            assert np.array_equal(prof[oo[s]:oo[s + 1]], p1), s
+            assert np.array_equal(idx[oo[s]:oo[s + 1]], i1), s
+
+
 def test_batch_within_fp32_tolerance_of_exact():
     lengths, windows, series = make_batch(4, 12)
     with tc.Device(0) as dev:
