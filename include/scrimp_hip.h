@@ -177,10 +177,11 @@ int mp_batch_destroy(mp_batch* batch);
 /* ---- serving: continuous batching of self-join requests --------------------
  * mp_server_create starts an engine thread on the context's device with an
  * HBM budget for resident request state (64 bytes per point: series,
- * statistics, recurrence-term record, partial-profile keys, outputs).  Requests are
- * admitted in arrival order while they fit; each engine tick launches the
- * FP32-reseeded tile kernel once over a share of the remaining tiles of every
- * admitted request (about tick_cells cells per tick; <= 0 picks 4e9), so
+ * statistics, recurrence-term record, partial-profile keys, outputs).
+ * Requests are admitted in arrival order while they fit; each engine tick
+ * launches the FP32-reseeded tile kernel once over a share of the remaining
+ * tiles of every admitted request (about tick_cells cells per tick; <= 0 picks
+ * 4e9), so
  * requests that arrive while others run join the next launch.  A request's
  * profile/index (P = n-m+1 entries each, host memory the caller keeps alive
  * until completion) are written when it completes.  opts: exclusion, band_k,
