@@ -80,3 +80,41 @@ def test_bad_stride_is_an_error():
     A = tc.random_walk(1, 500)
     with pytest.raises(tc.ScrimpError):
         tc.prescrimp_f64(A, 32, 0.0)
+
+
+def test_full_size_properties():
+    # n = 2^20, m = 256, stride 0.25 (the measured workload): too large for the
+    # oracle's PreSCRIMP, so size-independent properties are checked instead.
+    n, m, stride = 1 << 20, 256, 0.25
+    A = tc.random_walk(1, n)
+    print("prescrimp n=2^20 m=256", flush=True)
+    p, i, tm = tc.prescrimp_f64(A, m, stride)
+    P = n - m + 1
+    excl = m // 4
+    step = int(np.floor(stride * m))
+    assert p.size == P and tm["work_items"] == (P + step - 1) // step
+    assert np.all(np.isfinite(p))
+    assert np.all(np.abs(np.arange(P) - i) >= excl)
+    # Tolerance: the window statistics follow the reference's prefix-sum
+    # formula, whose sums of A^2 reach ~5e11 on this walk; their rounding
+    # leaves ~1e-8 relative error in a window's variance, ~1e-5 in d2 (for
+    # n <= 20000 the tests above hold 1e-9 * (d2 + 2m)).
+    def tol(d):
+        return 1e-7 * (d + 2 * m)
+
+    # every value is the distance to its reported partner
+    rng = np.random.default_rng(7)
+    for r in rng.choice(P, 200, replace=False):
+        d = orc.pair_dist2(A, m, int(r), int(i[r]))
+        assert abs(p[r] - d) <= tol(d), (r, p[r], d)
+    # sampled rows hold their full row minimum (long-double window moments,
+    # direct sliding dot products)
+    mu, sg = orc.window_moments_ld(A, m)
+    for r in (0, step * 4093, step * ((P - 1) // step)):
+        x = A[r:r + m] - mu[r]
+        c = np.correlate(A, x, mode="valid") - mu * x.sum()
+        d2 = 2.0 * m * (1.0 - c / (m * sg[r] * sg))
+        d2[max(0, r - excl + 1):r + excl] = np.inf
+        best = d2.min()
+        assert abs(p[r] - best) <= tol(best), (r, p[r], best)
+        assert d2[i[r]] <= best + tol(best)
