@@ -59,6 +59,22 @@ def test_reseed_interval_variants(tile_len, accum64):
     assert erru.max() <= 1e-2
 
 
+def test_config2_full_size_within_tolerance_of_fp64():
+    # n = 2^20 random walk (seed 1), m = 256: the measured workload, default tiles
+    A = tc.random_walk(1, 1 << 20)
+    m = 256
+    p64, i64, _ = tc.selfjoin_f64(A, m)
+    for acc in (True, False):
+        p, i, tm = tc.selfjoin_f32r(A, m, accum64=acc)
+        err = rel_err(p, p64, m)
+        print(f"accum64={acc} T={tm['tile_len']} max_rel={err.max():.3e} mean_rel={err.mean():.3e} "
+              f"idx_equal={np.mean(i == i64):.5f}")
+        assert err.max() <= (F64_ACC_MAX_REL if acc else F32_ACC_MAX_REL)
+        assert err.mean() <= 1e-6
+        assert np.mean(i == i64) >= 0.999
+        assert np.all(np.abs(np.arange(p.size) - i) > m // 4)
+
+
 def test_fp64_accumulation_removes_the_drift():
     # FP64 accumulation folds each 16-step FP32 partial into an FP64 sum; without
     # the folds its partial would span the tile and drift like FP32 accumulation.
