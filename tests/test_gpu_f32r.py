@@ -75,6 +75,28 @@ def test_config2_full_size_within_tolerance_of_fp64():
         assert np.all(np.abs(np.arange(p.size) - i) > m // 4)
 
 
+def test_config3_north_star_precision():
+    # n = 2^22, m = 1024, the precision workload (walk + 3 sinusoids + 20 bumps,
+    # seed 2): the FP32-reseeded path with its defaults within 1e-4 of FP64
+    n, m = 1 << 22, 1024
+    A, _ = tc.precision_series(2, n, m)
+    print("cfg3 exact", flush=True)
+    p64, i64, _ = tc.selfjoin_f64(A, m)
+    print("cfg3 f32r", flush=True)
+    p, i, tm = tc.selfjoin_f32r(A, m)
+    err = rel_err(p, p64, m)
+    print(f"T={tm['tile_len']} max_rel={err.max():.3e} mean_rel={err.mean():.3e} idx_equal={np.mean(i == i64):.5f}")
+    assert err.max() <= 1e-4
+    assert err.mean() <= 1e-6
+    assert np.mean(np.abs(i - i64) <= m // 4) >= 0.999
+    # the exact path's values are distances to their partners (long double);
+    # the prefix-sum statistics of a 2^22-point series allow ~1e-7 of d2 + 2m
+    rng = np.random.default_rng(11)
+    for r in rng.choice(p64.size, 50, replace=False):
+        d = orc.pair_dist2(A, m, int(r), int(i64[r]))
+        assert abs(p64[r] - d) <= 1e-7 * (d + 2 * m), (r, p64[r], d)
+
+
 def test_fp64_accumulation_removes_the_drift():
     # FP64 accumulation folds each 16-step FP32 partial into an FP64 sum; without
     # the folds its partial would span the tile and drift like FP32 accumulation.
