@@ -59,6 +59,40 @@ def test_batch_of_short_series_matches_single_series():
             assert np.array_equal(idx[oo[s]:oo[s + 1]], i1), s
 
 
+def test_full_4096_series_batch_sample_matches_single_series():
+    # the measured batch (4,096 walks, n log-uniform in [2^10, 2^17], m in
+    # {32, 64, 128, 256}; FP32-representable values), default band and tiles,
+    # replayed from its hipGraph; 24 series checked against their own runs
+    count = 4096
+    u = tc.uniforms(3, count, stream=0)
+    v = tc.uniforms(3, count, stream=1)
+    lengths = np.floor(2.0 ** (10.0 + 7.0 * u)).astype(np.int64)
+    windows = np.array([32, 64, 128, 256], dtype=np.int32)[np.minimum((4 * v).astype(int), 3)]
+    series = [tc.random_walk(3, int(n), stream=100 + s).astype(np.float32).astype(np.float64)
+              for s, n in enumerate(lengths)]
+    with tc.Device(0) as dev:
+        with tc.Batch(dev, lengths, windows) as b:
+            d_ts = dev.upload(np.concatenate(series))
+            P = int(b.out_offsets[-1])
+            d_p = dev.alloc(8 * P)
+            d_i = dev.alloc(8 * P)
+            b.run(d_ts, d_p, d_i, use_graph=True)
+            tm = b.run(d_ts, d_p, d_i, use_graph=True)
+            prof = dev.download(d_p, np.empty(P))
+            idx = dev.download(d_i, np.empty(P, dtype=np.int64))
+            info = b.info()
+            oo = b.out_offsets
+    print(f"tiles={info['tiles']} band_k={info['band_k']} T={info['tile_len']} ms={tm['total_ms']:.1f}")
+    assert tm["cells"] == sum(tc.cells(int(n), int(m)) for n, m in zip(lengths, windows))
+    rng = np.random.default_rng(5)
+    picks = list(rng.choice(count, 22, replace=False)) + [int(np.argmax(lengths)), int(np.argmin(lengths))]
+    for s in picks:
+        p1, i1, _ = tc.selfjoin_f32r(series[s], int(windows[s]), band_k=info["band_k"],
+                                     tile_len=info["tile_len"])
+        assert np.array_equal(prof[oo[s]:oo[s + 1]], p1), s
+        assert np.array_equal(idx[oo[s]:oo[s + 1]], i1), s
+
+
 def test_batch_within_fp32_tolerance_of_exact():
     lengths, windows, series = make_batch(4, 12)
     with tc.Device(0) as dev:
