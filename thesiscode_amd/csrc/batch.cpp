@@ -297,9 +297,22 @@ int mp_batch_create(mp_ctx* ctx, int32_t nseries, const int64_t* lengths, const 
     b->excl = o ? o->exclusion : -1;
     if (o && o->exclusion < -1) return fail(MP_ERR_BAD_WINDOW);
     int bk = (o && o->band_k > 0) ? o->band_k : 8;
-    int tl = (o && o->tile_len > 0) ? o->tile_len : 1024;
-    int rc = b->plan.build(nseries, lengths, windows, b->excl, bk, tl);
-    if (rc != MP_OK) return fail(rc);
+    int tl = (o && o->tile_len > 0) ? o->tile_len : 0;
+    int rc = MP_OK;
+    if (tl > 0) {
+        rc = b->plan.build(nseries, lengths, windows, b->excl, bk, tl);
+        if (rc != MP_OK) return fail(rc);
+    } else {
+        // the longest tile (8192 ... 512) that still gives every SIMD four
+        // tiles' worth of waves, as for one series
+        const int64_t target = 16LL * std::max(1, ctx ? ctx->cus : 256);
+        for (int T = 8192;; T >>= 1) {
+            rc = b->plan.build(nseries, lengths, windows, b->excl, bk, T);
+            if (rc != MP_OK) return fail(rc);
+            tl = T;
+            if (b->plan.ntiles >= target || T <= 512) break;
+        }
+    }
     // a cached tuning result replaces the defaults the caller left open
     if (!(o && o->band_k > 0) || !(o && o->tile_len > 0)) {
         int cbk, ctl;
