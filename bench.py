@@ -352,7 +352,7 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
-    ap.add_argument("--serve-tick-cells", type=float, default=4e9)
+    ap.add_argument("--serve-tick-cells", type=float, default=1.5e9)
     ap.add_argument("--no-variant", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 1 GiB device to device 3 times (calibration of rocprof byte counters)")

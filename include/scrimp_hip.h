@@ -181,7 +181,7 @@ int mp_batch_destroy(mp_batch* batch);
  * Requests are admitted in arrival order while they fit; each engine tick
  * launches the FP32-reseeded tile kernel once over a share of the remaining
  * tiles of every admitted request (about tick_cells cells per tick; <= 0 picks
- * 4e9), so
+ * 1.5e9), so
  * requests that arrive while others run join the next launch.  A request's
  * profile/index (P = n-m+1 entries each, host memory the caller keeps alive
  * until completion) are written when it completes.  opts: exclusion, band_k,

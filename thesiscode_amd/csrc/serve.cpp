@@ -567,7 +567,9 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     s->index_init = (o && o->index_init == 0) ? 0 : -1;
     s->excl = o ? o->exclusion : -1;
     const double tile_cells = 64.0 * s->band_k * s->tile_len;
-    const double tc = tick_cells > 0 ? static_cast<double>(tick_cells) : 4e9;
+    // 1.5e9 cells (about 0.3 ms of the GPU): measured p99 11 ms at 2,000 requests/s and
+    // 36 ms at 80% of the batched throughput, against 19 and 53 ms with 4e9
+    const double tc = tick_cells > 0 ? static_cast<double>(tick_cells) : 1.5e9;
     s->tick_tiles = std::max<int64_t>(1, static_cast<int64_t>(tc / tile_cells));
     // device pools: A, mu, sigma, profile, keys, index and a term record per point
     const int64_t per_elem = 8 * 6 + static_cast<int64_t>(mp::term_rec_bytes());
