@@ -337,6 +337,11 @@ __device__ __forceinline__ void walk_body(
         double v[kWave / 4];
 #pragma unroll
         for (int u = 0; u < kWave / 4; ++u) v[u] = pv[u];
+        if constexpr (RING) {  // the filter values, read behind the partials: one LDS wait for both
+            __builtin_amdgcn_sched_barrier(0);
+            pc = filt[lane & (F - 1)];
+            pr = filt[F + (lane & (F - 1))];
+        }
         const double thr = unord_f64((pc >> ib) << ib);
         bool cand = false;
 #pragma unroll
@@ -591,8 +596,6 @@ __device__ __forceinline__ void walk_body(
                     // the next rows) must have landed: the compiler does not
                     // order LDS reads after them
                     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                    pc = filt[lane & (F - 1)];
-                    pr = filt[F + (lane & (F - 1))];
                     flush(c0, F, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
                 } else {
                     flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
@@ -603,11 +606,7 @@ __device__ __forceinline__ void walk_body(
             // the last partial window
             if (c0 < L) {
                 window(c0, std::true_type{});
-                if constexpr (RING) {
-                    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                    pc = filt[lane & (F - 1)];
-                    pr = filt[F + (lane & (F - 1))];
-                }
+                if constexpr (RING) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
             }
         }
