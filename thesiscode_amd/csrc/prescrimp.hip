@@ -39,6 +39,13 @@ __device__ __forceinline__ SP pick(SP a, SP b) {
     return take ? b : a;
 }
 
+// pick() within a lane, whose candidates arrive in increasing partner order:
+// a strict comparison keeps the first maximum, so no tie test is needed
+__device__ __forceinline__ SP pick_ordered(SP a, SP b) {
+    const bool take = b.s > a.s || (a.s != a.s);
+    return take ? b : a;
+}
+
 // pick() over lanes l, l^16, l^32, l^48 (gfx950 row swaps; pick is symmetric
 // up to NaN pairs, which are discarded)
 __device__ __forceinline__ SP pick_quarters(SP x) {
@@ -231,8 +238,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     const bool ok = rv && j < P && !(j > i - excl && j < i + excl);
                     const double sc = fma(acc[b][c][r], rs * cs[c], -(rm * cm[c]));
                     const double s = ok ? sc : __builtin_nan("");
-                    rbest[b][r] = pick(rbest[b][r], SP{s, j});
-                    cbest[c] = pick(cbest[c], SP{s, i});
+                    // j rises along a lane's columns and i along its rows
+                    rbest[b][r] = pick_ordered(rbest[b][r], SP{s, j});
+                    cbest[c] = pick_ordered(cbest[c], SP{s, i});
                 }
             }
         }
