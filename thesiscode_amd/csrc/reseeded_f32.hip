@@ -561,14 +561,14 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
             const double mi = (p >= 0) ? mu[ai] : 0.0, mp = (p >= 0) ? mu[ap] : 0.0;
             double mine = 0.0;
             for (int u = 0; u < kUnpackChunk / 4; ++u) {
-                const int q = 4 * u + row;  // the entry this row sums in round u
+                const int q = 4 * u + row;  // the entry this row sums in pass u
                 const int qm = __shfl(need, q);
                 const int64_t qi = __shfl(ai, q), qp = __shfl(ap, q);
                 const double qmi = __shfl(mi, q), qmp = __shfl(mp, q);
                 double acc = 0.0;
                 for (int t = rl; t < qm; t += 16) acc = fma(A[qi + t] - qmi, A[qp + t] - qmp, acc);
                 acc = row_sum16(acc);
-                // entry `lane` was summed by row lane % 4 in round lane / 4
+                // entry `lane` was summed by row lane % 4 in pass lane / 4
                 const double got = __shfl(acc, 16 * (lane % 4));
                 if (lane / 4 == u) mine = got;
             }
