@@ -100,9 +100,10 @@ def test_flat_windows_follow_reference_semantics():
 
 
 def _block_inputs(seed, n, m, base):
-    # one extra 0.0 after the series: the last update of every diagonal reads
-    # A_vert[trianglen-1+w], one past the series (kernels.cpp:218-222)
-    A = np.append(tc.random_walk(seed, n), 0.0)
+    # one extra element after the series: the last update of every diagonal
+    # reads A_vert[trianglen-1+w], one past the series (kernels.cpp:218-222); it
+    # is not 0, so the written-back tmpQ shows that the drop-in reads it too
+    A = np.append(tc.random_walk(seed, n), 3.75)
     s, mu = orc.window_stats(A[:n], m)
     qt = orc.seeds(A[:n], m, base, n - m)
     return A, s, mu, qt

@@ -331,61 +331,58 @@ int mp_selfjoin_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, dou
     return selfjoin_f64_locked(c, d_ts, g, d_profile, d_index, o, tm);
 }
 
-// Host-pointer wrapper shared by both self-joins.
-using DevSelfJoin = int (*)(mp_ctx*, const double*, int64_t, int32_t, double*, int64_t*,
-                            const mp_opts*, mp_timing*);
+// Host-pointer wrapper shared by the self-joins and PreSCRIMP.  The default
+// context's lock is held for the whole call (staging, compute, copy-back): the
+// staging arena is shared by every host-pointer call on the context, so a
+// second caller must not reserve (and possibly free) or re-use it while the
+// first one still computes on it or copies from it.
+}  // extern "C"
 
-static int host_selfjoin(DevSelfJoin fn, const double* ts, int64_t n, int32_t m, double* profile,
-                         int64_t* index, const mp_opts* o, mp_timing* tm) {
+template <class Fn>
+static int host_call(const double* ts, int64_t n, const Geometry& g, double* profile, int64_t* index,
+                     Fn&& locked_fn) {
     if (!ts || !profile || !index) return MP_ERR_NULL;
-    Geometry g;
-    MP_TRY(geometry(n, m, o, &g));
     mp_ctx* c = nullptr;
     MP_TRY(default_ctx(&c));
-    double* dts;
-    double* dprof;
-    int64_t* didx;
-    {
-        std::lock_guard<std::mutex> guard(c->lock);
-        MP_HIP(hipSetDevice(c->device));
-        MP_TRY(c->io.reserve(aligned_bytes(n, 8) + 2 * aligned_bytes(g.P, 8)));
-        c->io.used = 0;
-        dts = c->io.take<double>(n);
-        dprof = c->io.take<double>(g.P);
-        didx = c->io.take<int64_t>(g.P);
-        MP_HIP(hipMemcpyAsync(dts, ts, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    }
-    MP_TRY(fn(c, dts, n, m, dprof, didx, o, tm));
     std::lock_guard<std::mutex> guard(c->lock);
+    MP_HIP(hipSetDevice(c->device));
+    MP_TRY(c->io.reserve(aligned_bytes(n, 8) + 2 * aligned_bytes(g.P, 8)));
+    c->io.used = 0;
+    double* dts = c->io.take<double>(n);
+    double* dprof = c->io.take<double>(g.P);
+    int64_t* didx = c->io.take<int64_t>(g.P);
+    MP_HIP(hipMemcpyAsync(dts, ts, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    MP_TRY(locked_fn(c, static_cast<const double*>(dts), dprof, didx));
     MP_HIP(hipMemcpyAsync(profile, dprof, sizeof(double) * g.P, hipMemcpyDeviceToHost, c->stream));
     MP_HIP(hipMemcpyAsync(index, didx, sizeof(int64_t) * g.P, hipMemcpyDeviceToHost, c->stream));
     MP_HIP(hipStreamSynchronize(c->stream));
     return MP_OK;
 }
 
+extern "C" {
+
 int mp_selfjoin_f64(const double* ts, int64_t n, int32_t m, double* profile, int64_t* index,
                     const mp_opts* o, mp_timing* tm) {
-    return host_selfjoin(mp_selfjoin_f64_dev, ts, n, m, profile, index, o, tm);
+    Geometry g;
+    MP_TRY(geometry(n, m, o, &g));
+    return host_call(ts, n, g, profile, index, [&](mp_ctx* c, const double* dts, double* dp, int64_t* di) {
+        return selfjoin_f64_locked(c, dts, g, dp, di, o, tm);
+    });
 }
 
 // ---------------------------------------------------------------------------
 // PreSCRIMP (the anytime phase of scrimppp_orig)
 // ---------------------------------------------------------------------------
-int mp_prescrimp_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double stride,
-                         double* d_profile, int64_t* d_index, mp_timing* tm) {
-    if (!c || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
-    if (!(stride > 0.0) || !(stride < 1e9)) return MP_ERR_BAD_ARG;
-    Geometry g;
-    MP_TRY(geometry(n, m, nullptr, &g));
-    const int64_t P = g.P;
+static int prescrimp_locked(mp_ctx* c, const double* d_ts, const Geometry& g, double stride,
+                            double* d_profile, int64_t* d_index, mp_timing* tm) {
+    const int64_t n = g.n, P = g.P;
+    const int m = g.m;
     // the reference's stepSize = floor(stride*m) (ScrimpppOrig.cpp:51), at least 1
     const int64_t step = std::max<int64_t>(1, static_cast<int64_t>(std::floor(stride * m)));
     const int64_t nrows = (P + step - 1) / step;
-    std::lock_guard<std::mutex> guard(c->lock);
-    MP_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n, 8) + 2 * aligned_bytes(P + 64, 8) +
-                        2 * aligned_bytes(P, 8);
+                        2 * aligned_bytes(P, 8) + aligned_bytes(nrows, 4);
     MP_TRY(c->work.reserve(need));
     c->work.used = 0;
     double* A = c->work.take<double>(n + 64);
@@ -395,6 +392,7 @@ int mp_prescrimp_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     double* mus = c->work.take<double>(P + 64);
     auto* val = c->work.take<unsigned long long>(P);
     auto* key = c->work.take<unsigned long long>(P);
+    auto* partner = c->work.take<int>(nrows);
     const int ib = bits_for(P + 1);
     const unsigned long long v0 = mp::ord_f64_host(-std::numeric_limits<double>::infinity());
     MP_HIP(hipEventRecord(c->ev[0], st));
@@ -417,6 +415,7 @@ int mp_prescrimp_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     a.cols_per_wave = 1024;
     a.val = val;
     a.key = key;
+    a.partner = partner;
     MP_TRY(mp::launch_prescrimp(a, st));
     MP_HIP(hipEventRecord(c->ev[3], st));
     // untouched entries keep d2 = +inf and index 0, as the reference initialises them
@@ -435,45 +434,39 @@ int mp_prescrimp_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     return MP_OK;
 }
 
-int mp_prescrimp_f64(const double* ts, int64_t n, int32_t m, double stride, double* profile,
-                     int64_t* index, mp_timing* tm) {
-    if (!ts || !profile || !index) return MP_ERR_NULL;
+int mp_prescrimp_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double stride,
+                         double* d_profile, int64_t* d_index, mp_timing* tm) {
+    if (!c || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
+    if (!(stride > 0.0) || !(stride < 1e9)) return MP_ERR_BAD_ARG;
     Geometry g;
     MP_TRY(geometry(n, m, nullptr, &g));
-    mp_ctx* c = nullptr;
-    MP_TRY(default_ctx(&c));
-    double* dts;
-    double* dprof;
-    int64_t* didx;
-    {
-        std::lock_guard<std::mutex> guard(c->lock);
-        MP_HIP(hipSetDevice(c->device));
-        MP_TRY(c->io.reserve(aligned_bytes(n, 8) + 2 * aligned_bytes(g.P, 8)));
-        c->io.used = 0;
-        dts = c->io.take<double>(n);
-        dprof = c->io.take<double>(g.P);
-        didx = c->io.take<int64_t>(g.P);
-        MP_HIP(hipMemcpyAsync(dts, ts, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    }
-    MP_TRY(mp_prescrimp_f64_dev(c, dts, n, m, stride, dprof, didx, tm));
     std::lock_guard<std::mutex> guard(c->lock);
-    MP_HIP(hipMemcpyAsync(profile, dprof, sizeof(double) * g.P, hipMemcpyDeviceToHost, c->stream));
-    MP_HIP(hipMemcpyAsync(index, didx, sizeof(int64_t) * g.P, hipMemcpyDeviceToHost, c->stream));
-    MP_HIP(hipStreamSynchronize(c->stream));
-    return MP_OK;
+    MP_HIP(hipSetDevice(c->device));
+    return prescrimp_locked(c, d_ts, g, stride, d_profile, d_index, tm);
+}
+
+int mp_prescrimp_f64(const double* ts, int64_t n, int32_t m, double stride, double* profile,
+                     int64_t* index, mp_timing* tm) {
+    if (!(stride > 0.0) || !(stride < 1e9)) return MP_ERR_BAD_ARG;
+    Geometry g;
+    MP_TRY(geometry(n, m, nullptr, &g));
+    return host_call(ts, n, g, profile, index, [&](mp_ctx* c, const double* dts, double* dp, int64_t* di) {
+        return prescrimp_locked(c, dts, g, stride, dp, di, tm);
+    });
 }
 
 // ---------------------------------------------------------------------------
 // FP32-reseeded self-join (a batch of one series)
 // ---------------------------------------------------------------------------
-// Columns per tile (the re-seed interval): the longest of 8192..512 that still
-// gives every SIMD of the device 4 tiles' worth of waves.  Longer tiles amortise
-// the FP64 seed (m FMAs per diagonal) over more cells.
-static int default_tile_len(int64_t W, int band_k, int cus, int requested) {
+// Columns per tile (the re-seed interval): the longest of max_auto_tile_len(accum64)
+// ... 512 that still gives every SIMD of the device 4 tiles' worth of waves.
+// Longer tiles amortise the FP64 seed (m FMAs per diagonal) over more cells; the
+// cap keeps an automatic choice inside the accuracy the accumulation mode holds.
+static int default_tile_len(int64_t W, int band_k, int cus, int requested, int accum64) {
     if (requested > 0) return requested;
     const int64_t bw = 64LL * band_k;
     const int64_t target = 16LL * cus;
-    for (int T = 8192; T >= 512; T >>= 1) {
+    for (int T = mp::max_auto_tile_len(accum64); T >= 512; T >>= 1) {
         int64_t nt = 0;
         for (int64_t d = 0; d < W; d += bw) nt += (W - d + T - 1) / T;
         if (nt >= target) return T;
@@ -481,17 +474,16 @@ static int default_tile_len(int64_t W, int band_k, int cus, int requested) {
     return 256;
 }
 
-int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double* d_profile,
-                         int64_t* d_index, const mp_opts* o, mp_timing* tm) {
-    if (!c || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
-    Geometry g;
-    MP_TRY(geometry(n, m, o, &g));
+static int selfjoin_f32r_locked(mp_ctx* c, const double* d_ts, const Geometry& g, double* d_profile,
+                                int64_t* d_index, const mp_opts* o, mp_timing* tm) {
+    const int64_t n = g.n;
+    const int m = g.m;
     const int band_k = (o && o->band_k > 0) ? o->band_k : 8;
     if (band_k != 4 && band_k != 8 && band_k != 16) return MP_ERR_BAD_ARG;
-    const int T = default_tile_len(g.W, band_k, c->cus, o ? o->tile_len : 0);
-    if (T % 16 != 0 || T <= 0) return MP_ERR_BAD_ARG;
     const int refine = o ? (o->refine != 0) : 1;
     const int accum64 = o ? (o->accum64 != 0) : 1;
+    const int T = default_tile_len(g.W, band_k, c->cus, o ? o->tile_len : 0, accum64);
+    if (T % 16 != 0 || T <= 0) return MP_ERR_BAD_ARG;
     const int64_t index_init = (o && o->index_init == 0) ? 0 : -1;
     const int BW = 64 * band_k;
     const int nbands = (g.W + BW - 1) / BW;
@@ -516,8 +508,6 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     const int64_t sprefix[2] = {0, mp::stat_chunks(static_cast<int>(g.P))};
     const int64_t tprefix[2] = {0, ntiles};
 
-    std::lock_guard<std::mutex> guard(c->lock);
-    MP_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n + 64, 8) +
                         aligned_bytes(n + 64, mp::term_rec_bytes()) + aligned_bytes(n + 64, 8) +
@@ -592,9 +582,23 @@ int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, do
     return MP_OK;
 }
 
+int mp_selfjoin_f32r_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double* d_profile,
+                         int64_t* d_index, const mp_opts* o, mp_timing* tm) {
+    if (!c || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
+    Geometry g;
+    MP_TRY(geometry(n, m, o, &g));
+    std::lock_guard<std::mutex> guard(c->lock);
+    MP_HIP(hipSetDevice(c->device));
+    return selfjoin_f32r_locked(c, d_ts, g, d_profile, d_index, o, tm);
+}
+
 int mp_selfjoin_f32r(const double* ts, int64_t n, int32_t m, double* profile, int64_t* index,
                      const mp_opts* o, mp_timing* tm) {
-    return host_selfjoin(mp_selfjoin_f32r_dev, ts, n, m, profile, index, o, tm);
+    Geometry g;
+    MP_TRY(geometry(n, m, o, &g));
+    return host_call(ts, n, g, profile, index, [&](mp_ctx* c, const double* dts, double* dp, int64_t* di) {
+        return selfjoin_f32r_locked(c, dts, g, dp, di, o, tm);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -668,11 +672,14 @@ int mp_eval_diag_block_triangle_f64(double prof_colmin[], int64_t idx_colmin[],
         idxout[s] = c->work.take<int64_t>(span);
     }
     auto* counter = c->work.take<unsigned int>(1);
-    // A_hor: elements 0..tl-1+w are read; A_vert: 0..tl-2+w plus the element
-    // the reference reads one past at its very last update, which is replaced by 0.
+    // A_hor and A_vert: elements 0..tl-1+w are read.  A_vert[tl-1+w] enters only
+    // the last tmpQ update of each diagonal (kernels.cpp:218-222), after its last
+    // score: in the reference's self-join adapters it lies one past the series.
+    // It is copied as the caller holds it, so the written-back tmpQ equals the
+    // reference's for any value there.
     MP_HIP(hipMemcpyAsync(dAh, A_hor, sizeof(double) * na, hipMemcpyHostToDevice, st));
-    MP_HIP(hipMemcpyAsync(dAv, A_vert, sizeof(double) * (na - 1), hipMemcpyHostToDevice, st));
-    MP_TRY(mp::launch_fill_u64(reinterpret_cast<unsigned long long*>(dAv + na - 1), 2, 0ull, st));
+    MP_HIP(hipMemcpyAsync(dAv, A_vert, sizeof(double) * na, hipMemcpyHostToDevice, st));
+    MP_TRY(mp::launch_fill_u64(reinterpret_cast<unsigned long long*>(dAv + na), 1, 0ull, st));
     MP_HIP(hipMemcpyAsync(dsh, s_hor, sizeof(double) * tl, hipMemcpyHostToDevice, st));
     MP_HIP(hipMemcpyAsync(dmh, mu_hor, sizeof(double) * tl, hipMemcpyHostToDevice, st));
     MP_HIP(hipMemcpyAsync(dsv, s_vert, sizeof(double) * tl, hipMemcpyHostToDevice, st));

@@ -303,10 +303,10 @@ int mp_batch_create(mp_ctx* ctx, int32_t nseries, const int64_t* lengths, const 
         rc = b->plan.build(nseries, lengths, windows, b->excl, bk, tl);
         if (rc != MP_OK) return fail(rc);
     } else {
-        // the longest tile (8192 ... 512) that still gives every SIMD four
-        // tiles' worth of waves, as for one series
-        const int64_t target = 16LL * std::max(1, ctx ? ctx->cus : 256);
-        for (int T = 8192;; T >>= 1) {
+        // the longest tile (max_auto_tile_len ... 512) that still gives every
+        // SIMD four tiles' worth of waves, as for one series
+        const int64_t target = 16LL * std::max(1, b->cus);
+        for (int T = mp::max_auto_tile_len(b->accum64);; T >>= 1) {
             rc = b->plan.build(nseries, lengths, windows, b->excl, bk, T);
             if (rc != MP_OK) return fail(rc);
             tl = T;
@@ -316,7 +316,7 @@ int mp_batch_create(mp_ctx* ctx, int32_t nseries, const int64_t* lengths, const 
     // a cached tuning result replaces the defaults the caller left open
     if (!(o && o->band_k > 0) || !(o && o->tile_len > 0)) {
         int cbk, ctl;
-        if (tune_lookup(tune_key(b), &cbk, &ctl)) {
+        if (tune_lookup(tune_key(b), &cbk, &ctl) && ctl <= mp::max_auto_tile_len(b->accum64)) {
             if (!(o && o->band_k > 0)) bk = cbk;
             if (!(o && o->tile_len > 0)) tl = ctl;
             rc = b->plan.build(nseries, lengths, windows, b->excl, bk, tl);
@@ -400,7 +400,10 @@ int mp_batch_run(mp_batch* b, const double* d_ts, double* d_profile, int64_t* d_
 int mp_batch_tune(mp_batch* b, const double* d_ts, double* d_profile, int64_t* d_index, int32_t max_tile_len,
                   mp_timing* best) {
     if (!b || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
+    // the tuner ranks by speed only among tile lengths whose accuracy the
+    // accumulation mode holds (max_auto_tile_len)
     if (max_tile_len <= 0) max_tile_len = 8192;
+    max_tile_len = std::min(max_tile_len, mp::max_auto_tile_len(b->accum64));
     const int bks[] = {4, 8};
     const int tls[] = {256, 512, 1024, 2048, 4096, 8192};
     double best_rate = -1.0;

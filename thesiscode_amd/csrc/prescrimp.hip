@@ -276,15 +276,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // the start cell is recomputed in FP64 (the reference recovers it from the
 // row's distance); the walk then updates it like the reference:
 // QT += A[i+k+m-1]*A[j+k+m-1] - A[i+k-1]*A[j+k-1].
+// Each sampled row's best partner once every row's distance profile has been
+// merged and before any walk starts: a walk merges into rows that other walks
+// read, so reading the keys inside k_pre_walk could start a walk from another
+// row's (near-tied) update instead of the row's own minimum, which is where the
+// reference starts it (ScrimpppOrig.cpp:215-217).
+__global__ void k_pre_partners(const unsigned long long* __restrict__ key, int step, int nrows, int ib,
+                               int* __restrict__ partner) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    partner[r] = static_cast<int>(key_partner(key[static_cast<int64_t>(r) * step], ib));
+}
+
 __global__ void k_pre_walk(const double* __restrict__ A, const double* __restrict__ sInv,
                            const double* __restrict__ mus, int P, int m, int step, int nrows,
-                           int ib, unsigned long long* val, unsigned long long* key) {
+                           int ib, const int* __restrict__ partner, unsigned long long* val,
+                           unsigned long long* key) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
     const int i = r * step;
-    const int64_t pj = key_partner(key[i], ib);
-    if (pj < 0) return;
-    const int j = static_cast<int>(pj);
+    const int j = partner[r];
+    if (j < 0) return;
     double z0 = 0.0;
     for (int t = 0; t < m; ++t) z0 = fma(A[j + t], A[i + t], z0);
     double z = z0;
@@ -328,8 +340,10 @@ int launch_prescrimp(const PreScrimpLaunch& a, hipStream_t st) {
                        a.mus, a.P, a.m, a.step, a.nrows, a.excl, a.ib, a.cols_per_wave, ncolwaves,
                        a.val, a.key);
     if (hipGetLastError() != hipSuccess) return MP_ERR_DEVICE;
-    hipLaunchKernelGGL(k_pre_walk, dim3(static_cast<unsigned>((a.nrows + 255) / 256)), dim3(256), 0, st,
-                       a.A, a.sInv, a.mus, a.P, a.m, a.step, a.nrows, a.ib, a.val, a.key);
+    const dim3 rgrid(static_cast<unsigned>((a.nrows + 255) / 256));
+    hipLaunchKernelGGL(k_pre_partners, rgrid, dim3(256), 0, st, a.key, a.step, a.nrows, a.ib, a.partner);
+    hipLaunchKernelGGL(k_pre_walk, rgrid, dim3(256), 0, st, a.A, a.sInv, a.mus, a.P, a.m, a.step, a.nrows,
+                       a.ib, a.partner, a.val, a.key);
     return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
 }
 

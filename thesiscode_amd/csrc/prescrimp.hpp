@@ -17,6 +17,7 @@ struct PreScrimpLaunch {
     int ib;                  // partner bits of the merge keys
     int cols_per_wave;       // a multiple of 64
     unsigned long long *val, *key;  // P entries each, filled with ord(-inf)
+    int* partner;            // nrows entries: each sampled row's best partner before the walks
 };
 
 int launch_prescrimp(const PreScrimpLaunch& a, hipStream_t st);

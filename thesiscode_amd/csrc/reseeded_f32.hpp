@@ -65,6 +65,15 @@ __host__ __device__ inline int64_t band_tiles(int64_t L, int band_k, int T) {
     return (lm + T - 1) / T + 1;
 }
 
+// Longest re-seed interval (tile length) that the library, the batch autotuner
+// and the server pick by themselves.  With FP64 accumulation every interval up
+// to 8192 keeps the profile within 1e-4 of FP64 (n = 2^22, m = 1024 near-periodic
+// series: 4.3e-5 at every B).  FP32 accumulation drifts with the interval: on
+// random walks it stays within 1e-4 up to B = 1024 (6.5e-5) and leaves it from
+// B = 4096 on (1.5e-4), so automatic choices stop at 1024 there; a caller that
+// asks for a longer interval explicitly still gets it.
+inline int max_auto_tile_len(int accum64) { return accum64 ? 8192 : 1024; }
+
 // Window statistics are computed in chunks of kStatChunk windows of one series;
 // each chunk restarts its sums at its first window, so the statistics of a
 // series do not depend on what else the launch carries.

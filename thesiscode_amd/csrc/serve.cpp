@@ -170,7 +170,9 @@ struct mp_server {
     std::condition_variable cv_submit, cv_done;
     std::deque<Request*> waiting;
     std::vector<Request*> active;
-    std::unordered_map<uint64_t, std::unique_ptr<Request>> all;
+    // shared: a waiter keeps its request alive even when poll or completed
+    // hands the same request to another thread meanwhile
+    std::unordered_map<uint64_t, std::shared_ptr<Request>> all;
     std::vector<uint64_t> newly_done;  // completed ids not yet drained
     uint64_t next_id = 1;
     bool stop = false;
@@ -526,15 +528,18 @@ void engine_main(mp_server* s) {
             s->failed = rc;
         }
     }
-    // fail whatever is left
+    // fail whatever is left; the failed requests are reported by completed() too
     for (auto& kv : s->all) {
         Request* r = kv.second.get();
         if (r->state != DONE) {
             r->state = DONE;
             r->status = s->failed != MP_OK ? s->failed : MP_ERR_DEVICE;
             r->t_done = Clock::now();
+            s->newly_done.push_back(r->id);
         }
     }
+    s->waiting.clear();
+    s->active.clear();
     s->cv_done.notify_all();
 }
 
@@ -560,10 +565,10 @@ int mp_server_create(mp_ctx* ctx, uint64_t hbm_budget_bytes, int64_t tick_cells,
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) s->cus = prop.multiProcessorCount;
     s->band_k = (o && o->band_k > 0) ? o->band_k : 8;
-    s->tile_len = (o && o->tile_len > 0) ? o->tile_len : 4096;
-    if ((s->band_k != 4 && s->band_k != 8 && s->band_k != 16) || s->tile_len % 16 != 0) return fail(MP_ERR_BAD_ARG);
     s->refine = o ? (o->refine != 0) : 1;
     s->accum64 = o ? (o->accum64 != 0) : 1;
+    s->tile_len = (o && o->tile_len > 0) ? o->tile_len : std::min(4096, mp::max_auto_tile_len(s->accum64));
+    if ((s->band_k != 4 && s->band_k != 8 && s->band_k != 16) || s->tile_len % 16 != 0) return fail(MP_ERR_BAD_ARG);
     s->index_init = (o && o->index_init == 0) ? 0 : -1;
     s->excl = o ? o->exclusion : -1;
     const double tile_cells = 64.0 * s->band_k * s->tile_len;
@@ -642,7 +647,7 @@ int mp_server_submit(mp_server* s, const double* ts, int64_t n, int32_t m, doubl
     if (m < 1) return MP_ERR_BAD_WINDOW;
     if (n < m) return MP_ERR_SHORT_SERIES;
     if (n > (int64_t(1) << 30)) return MP_ERR_TOO_LONG;
-    auto r = std::make_unique<Request>();
+    auto r = std::make_shared<Request>();
     r->n = n;
     r->m = m;
     r->out_prof = profile_out;
@@ -686,11 +691,14 @@ int mp_server_wait(mp_server* s, uint64_t req_id, double* latency_ms) {
     std::unique_lock<std::mutex> lk(s->lock);
     auto it = s->all.find(req_id);
     if (it == s->all.end()) return MP_ERR_NOT_FOUND;
-    Request* r = it->second.get();
+    // hold the request itself: another thread's poll or completed may take the
+    // id out of the table while this one sleeps
+    const std::shared_ptr<Request> r = it->second;
     s->cv_done.wait(lk, [&] { return r->state == DONE; });
     const int st = r->status;
     if (latency_ms) *latency_ms = ms_since(r->t_submit, r->t_done);
-    s->all.erase(it);
+    auto again = s->all.find(req_id);
+    if (again != s->all.end() && again->second == r) s->all.erase(again);
     return st;
 }
 
