@@ -247,6 +247,33 @@ static void run_rank(rank_job* job) {
     job->seconds = now_s() - t0;
 }
 
+int orc_triv_par_rank(const double* A, long n, int m, long excl, int blocklen, int rank, int nranks,
+                      int kernel, int block_stride, double* profile, long* index, double* cells,
+                      double* seconds) {
+    const long P = n - m + 1;
+    if (P <= 0 || m < 1 || rank < 0 || rank >= nranks) return -1;
+    if (excl < 0) excl = m / 4;
+    if (g_fault == 4) excl = excl - 1;
+    rank_job job;
+    memset(&job, 0, sizeof job);
+    job.A = A;
+    job.n = n;
+    job.P = P;
+    job.excl = excl;
+    job.m = m;
+    job.blocklen = blocklen;
+    job.rank = rank;
+    job.nranks = nranks;
+    job.block_stride = block_stride;
+    job.kern = kernel ? orc_eval_block_abba : orc_eval_block_scalar;
+    job.profile = profile;
+    job.index = index;
+    run_rank(&job);
+    if (cells) *cells = job.cells;
+    if (seconds) *seconds = job.seconds;
+    return 0;
+}
+
 typedef struct {
     rank_job* jobs;
     int first_rank, stride, nranks;

@@ -66,6 +66,17 @@ int orc_triv_par(const double* A, long n, int m, long excl, int blocklen, int nr
                  double* cells, double* seconds);
 
 /*
+ * One rank's share of scrimp_triv_par (ScrimpTrivialPar.cpp:201-340): its own
+ * statistics, partitions `rank` and 2*nranks-1-rank, blocks with seeds and the
+ * kernel, the d2 transform; profile/index (P entries) are this rank's partial
+ * result, to be merged with the element-wise min reduction of
+ * ScrimpTrivialPar.cpp:68-108.  Used by the MPI launcher (one process per rank).
+ */
+int orc_triv_par_rank(const double* A, long n, int m, long excl, int blocklen, int rank, int nranks,
+                      int kernel, int block_stride, double* profile, long* index, double* cells,
+                      double* seconds);
+
+/*
  * scrimp_vert_blocked driver (ScrimpVertBlocked.cpp:89-186) without the
  * unseeded shuffle of block starts (order only changes ties): index starts
  * at 0 and profile at -1.0.

@@ -89,3 +89,122 @@ def test_poll_reports_progress_then_result():
             p, i, lat, q = res
             assert lat >= q >= 0.0
             assert np.all(np.abs(np.arange(p.size) - i) > 25)
+
+
+def test_config5_poisson_trace():
+    # The serving configuration, shortened to 5 s of arrivals: Poisson at 2,000
+    # requests/s, lengths lognormal (median 2^14, sigma 1) clipped to
+    # [2^10, 2^19], m uniform in {64, 128, 256, 512}, replayed in real time.
+    # Every request completes; a sample (the longest request among them) equals
+    # its own single FP32-reseeded run with the server's tiling bit for bit and
+    # the exact FP64 profile within 1e-4.
+    import threading
+
+    from thesiscode_amd.workloads import serve_requests
+
+    arrivals, lengths, windows, walk, offs = serve_requests(5, 5.0, 2000.0, walk_len=1 << 22)
+    count = len(lengths)
+    rng = np.random.default_rng(5)
+    sample = set(rng.choice(count, 19, replace=False).tolist()) | {int(np.argmax(lengths))}
+    print(f"cfg5 trace: {count} requests, longest {lengths.max()}, {lengths.sum()} points", flush=True)
+    kept, lat, errors = {}, {}, []
+    with tc.Device(0) as dev:
+        with tc.Server(dev, hbm_budget_bytes=16 << 30) as srv:
+            srv.wait(srv.submit(walk[:20000], 128))  # warm the kernels
+            ids = {}
+            t0 = time.perf_counter()
+
+            def submitter():
+                try:
+                    for k in range(count):
+                        delay = t0 + arrivals[k] - time.perf_counter()
+                        if delay > 0:
+                            time.sleep(delay)
+                        ids[srv.submit(walk[offs[k]:offs[k] + lengths[k]], int(windows[k]))] = k
+                except Exception as e:  # surfaced by the main thread
+                    errors.append(e)
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            deadline = t0 + 120.0
+            pending = {}
+            while (th.is_alive() or len(lat) < count) and not errors:
+                for rid, st, l_ms, _, p, i in srv.completed():
+                    assert st == 0, (rid, st)
+                    pending[rid] = (l_ms, p, i)
+                for rid in [r for r in pending if r in ids]:
+                    l_ms, p, i = pending.pop(rid)
+                    k = ids[rid]
+                    lat[k] = l_ms
+                    if k in sample:
+                        kept[k] = (p, i)
+                assert time.perf_counter() < deadline, f"{len(lat)}/{count} requests done"
+                time.sleep(0.002)
+            th.join()
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+    assert not errors, errors
+    assert len(lat) == count and st["completed"] == count + 1
+    lv = np.array(list(lat.values()))
+    cells = sum(tc.cells(int(n), int(m)) for n, m in zip(lengths, windows))
+    print(f"served {count} requests in {wall:.2f} s ({cells / wall:.3e} cells/s), "
+          f"p50 {np.percentile(lv, 50):.2f} ms, p99 {np.percentile(lv, 99):.2f} ms, ticks {st['ticks']}", flush=True)
+    for k in sorted(sample):
+        ts = walk[offs[k]:offs[k] + lengths[k]]
+        m = int(windows[k])
+        p, i = kept[k]
+        p1, i1, _ = tc.selfjoin_f32r(ts, m, band_k=8, tile_len=4096)
+        assert np.array_equal(p, p1) and np.array_equal(i, i1), k
+        p64, _, _ = tc.selfjoin_f64(ts, m)
+        assert np.max(np.abs(p - p64) / np.maximum(p64, 1e-3 * 2 * m)) <= 1e-4, k
+
+
+def test_wait_and_completed_on_two_threads():
+    # mp_server_wait on one thread while another drains mp_server_completed:
+    # every request is handed out (to the waiter, or to the drain when it took
+    # it first, then the waiter gets MP_ERR_NOT_FOUND), and results are intact.
+    import ctypes as C
+    import threading
+
+    from thesiscode_amd._ffi import lib
+
+    reqs = _requests(3, 40)
+    bufs = [(np.empty(ts.size - m + 1), np.empty(ts.size - m + 1, dtype=np.int64)) for ts, m in reqs]
+    with tc.Device(0) as dev:
+        with tc.Server(dev, hbm_budget_bytes=1 << 30) as srv:
+            h = srv._h
+            ids = []
+            for (ts, m), (p, i) in zip(reqs, bufs):
+                rid = C.c_uint64()
+                assert lib().mp_server_submit(h, ts.ctypes.data_as(C.POINTER(C.c_double)), ts.size, m,
+                                              p.ctypes.data_as(C.POINTER(C.c_double)),
+                                              i.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(rid)) == 0
+                ids.append(rid.value)
+            waited, drained = {}, set()
+
+            def waiter():
+                for rid in ids:
+                    waited[rid] = lib().mp_server_wait(h, rid, None)
+
+            def drainer():
+                ib = (C.c_uint64 * 64)()
+                sb = (C.c_int32 * 64)()
+                cnt = C.c_int32()
+                while len(waited) < len(ids):
+                    assert lib().mp_server_completed(h, 64, ib, sb, None, None, C.byref(cnt)) == 0
+                    for k in range(cnt.value):
+                        assert sb[k] == 0
+                        drained.add(ib[k])
+                    time.sleep(0.0005)
+
+            ths = [threading.Thread(target=waiter), threading.Thread(target=drainer)]
+            for t in ths:
+                t.start()
+            for t in ths:
+                t.join(120)
+            assert not any(t.is_alive() for t in ths)
+    for rid in ids:
+        assert waited[rid] == 0 or (waited[rid] == -10 and rid in drained), (rid, waited[rid])
+    for (ts, m), (p, i) in list(zip(reqs, bufs))[::8]:
+        p1, i1, _ = tc.selfjoin_f32r(ts, m, band_k=8, tile_len=4096)
+        assert np.array_equal(p, p1) and np.array_equal(i, i1)

@@ -139,6 +139,11 @@ class Device:
         self._bufs.append(p.value)
         return p.value
 
+    def free(self, ptr: int) -> None:
+        if ptr in self._bufs:
+            self._bufs.remove(ptr)
+            check(lib().mp_dev_free(self._ctx, C.c_void_p(ptr)), "mp_dev_free")
+
     def upload(self, arr: np.ndarray) -> int:
         arr = np.ascontiguousarray(arr)
         p = self.alloc(arr.nbytes)
