@@ -99,6 +99,24 @@ int mp_selfjoin_f64_dev(mp_ctx* ctx, const double* d_ts, int64_t n, int32_t m,
                         double* d_profile, int64_t* d_index, const mp_opts* opts,
                         mp_timing* timing);
 
+/* ---- the exact self-join without device allocation ------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory while they run (tight serving loops, code that must not call
+ * hipMalloc): the same computation and results as mp_selfjoin_f64_dev, bit for
+ * bit, in a file-scope device array of the library instead of the context's
+ * growable arena, so the call allocates and frees nothing.  The array holds
+ * series of up to 524,288 points (2^19, about 50 MB on every GPU the library is
+ * loaded on, kept until the process ends); a longer series returns
+ * MP_ERR_TOO_LONG.  All calls on one GPU share the array: they are serialised
+ * inside the library (a lock per GPU), and a call on another stream or process
+ * that overlaps one on the same GPU without this library's lock would race.
+ * The array keeps the last call's contents; every buffer in it is written
+ * before it is read.  Like mp_selfjoin_f64_dev it returns after the results
+ * are in d_profile/d_index.                                                 */
+int mp_selfjoin_f64_dev_noalloc(mp_ctx* ctx, const double* d_ts, int64_t n, int32_t m,
+                                double* d_profile, int64_t* d_index, const mp_opts* opts,
+                                mp_timing* timing);
+
 /* ---- PreSCRIMP, the anytime phase of scrimppp_orig --------------------------
  * Replaces the PreSCRIMP loop of ScrimpppOrig::compute_matrix_profile
  * (scrimppp/src/ScrimpppOrig.cpp:130-283): every sampled row i = 0, step,

@@ -156,7 +156,8 @@ class Device:
 
     def selfjoin(self, precision: str, d_ts: int, n: int, m: int, d_prof: int, d_idx: int,
                  opts: Opts | None = None) -> dict:
-        fn = {"f64": "mp_selfjoin_f64_dev", "f32r": "mp_selfjoin_f32r_dev"}[precision]
+        fn = {"f64": "mp_selfjoin_f64_dev", "f32r": "mp_selfjoin_f32r_dev",
+              "f64_noalloc": "mp_selfjoin_f64_dev_noalloc"}[precision]
         opts = opts or make_opts()
         tm = Timing()
         rc = getattr(lib(), fn)(self._ctx, C.c_void_p(d_ts), n, m, C.c_void_p(d_prof), C.c_void_p(d_idx),

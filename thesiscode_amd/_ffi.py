@@ -59,6 +59,8 @@ SIGNATURES = {
     "mp_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_uint64]),
     "mp_selfjoin_f64": (C.c_int, [_D, C.c_int64, C.c_int32, _D, _L, C.POINTER(Opts), C.POINTER(Timing)]),
     "mp_selfjoin_f64_dev": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P, C.POINTER(Opts), C.POINTER(Timing)]),
+    "mp_selfjoin_f64_dev_noalloc": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P, C.POINTER(Opts),
+                                              C.POINTER(Timing)]),
     "mp_prescrimp_f64": (C.c_int, [_D, C.c_int64, C.c_int32, C.c_double, _D, _L, C.POINTER(Timing)]),
     "mp_prescrimp_f64_dev": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_double, _P, _P, C.POINTER(Timing)]),
     "mp_selfjoin_f32r": (C.c_int, [_D, C.c_int64, C.c_int32, _D, _L, C.POINTER(Opts), C.POINTER(Timing)]),

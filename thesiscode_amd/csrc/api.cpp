@@ -45,6 +45,8 @@ int ctx_init(mp_ctx* c, int device) {
     c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     MP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) MP_HIP(hipEventCreate(&e));
+    // the allocation-free entry point's scratch address on this device
+    MP_TRY(mp::noalloc_scratch_init(device));
     return MP_OK;
 }
 
@@ -133,6 +135,18 @@ unsigned long long init_val_f64() {
     std::memcpy(&b, &x, sizeof b);
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
+
+// Buffers carved one after another from a scratch region (256-byte aligned).
+struct Carve {
+    char* base;
+    size_t used = 0;
+    template <class T>
+    T* take(int64_t count) {
+        T* p = reinterpret_cast<T*>(base + used);
+        used += mp::exact_aligned(count, sizeof(T));
+        return p;
+    }
+};
 
 }  // namespace
 
@@ -238,8 +252,10 @@ int mp_memcpy_d2h(mp_ctx* c, void* dst, const void* src, uint64_t bytes) {
 // ---------------------------------------------------------------------------
 // exact FP64 self-join
 // ---------------------------------------------------------------------------
-static int selfjoin_f64_locked(mp_ctx* c, const double* d_ts, const Geometry& g, double* d_profile,
-                               int64_t* d_index, const mp_opts* o, mp_timing* tm) {
+// The exact self-join on `scratch` (at least mp::exact_scratch_bytes(n, P, W)
+// bytes of device memory); every buffer is written before it is read.
+static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const Geometry& g,
+                            double* d_profile, int64_t* d_index, const mp_opts* o, mp_timing* tm) {
     hipStream_t st = c->stream;
     const int64_t n = g.n, P = g.P;
     const int m = g.m;
@@ -247,22 +263,17 @@ static int selfjoin_f64_locked(mp_ctx* c, const double* d_ts, const Geometry& g,
     const int W = g.W;
     const int band_k = pick_band_k_f64(c, W, o ? o->band_k : 0);
     if (band_k != 1 && band_k != 2 && band_k != 4 && band_k != 8) return MP_ERR_BAD_ARG;
-    const size_t need = aligned_bytes(n + 64, 8) + 2 * aligned_bytes(n, 8) +
-                        2 * aligned_bytes(P + 64, 8) + aligned_bytes(W + 1, 8) +
-                        aligned_bytes(P + mp::kWalkRecPad, sizeof(mp::WalkRec)) +
-                        2 * aligned_bytes(P, 8) + aligned_bytes(1, 4);
-    MP_TRY(c->work.reserve(need));
-    c->work.used = 0;
-    double* A = c->work.take<double>(n + 64);
-    double* cs = c->work.take<double>(n);
-    double* sq = c->work.take<double>(n);
-    double* sInv = c->work.take<double>(P + 64);
-    double* mus = c->work.take<double>(P + 64);
-    double* qt = c->work.take<double>(W + 1);
-    auto* rec = c->work.take<mp::WalkRec>(P + mp::kWalkRecPad);
-    auto* val = c->work.take<unsigned long long>(P);
-    auto* key = c->work.take<unsigned long long>(P);
-    auto* counter = c->work.take<unsigned int>(1);
+    Carve cv{scratch};
+    double* A = cv.take<double>(n + 64);
+    double* cs = cv.take<double>(n);
+    double* sq = cv.take<double>(n);
+    double* sInv = cv.take<double>(P + 64);
+    double* mus = cv.take<double>(P + 64);
+    double* qt = cv.take<double>(W + 1);
+    auto* rec = cv.take<mp::WalkRec>(P + mp::kWalkRecPad);
+    auto* val = cv.take<unsigned long long>(P);
+    auto* key = cv.take<unsigned long long>(P);
+    auto* counter = cv.take<unsigned int>(1);
     const int ib = bits_for(P + 1);
 
     MP_HIP(hipEventRecord(c->ev[0], st));
@@ -321,6 +332,13 @@ static int selfjoin_f64_locked(mp_ctx* c, const double* d_ts, const Geometry& g,
     return MP_OK;
 }
 
+static int selfjoin_f64_locked(mp_ctx* c, const double* d_ts, const Geometry& g, double* d_profile,
+                               int64_t* d_index, const mp_opts* o, mp_timing* tm) {
+    MP_TRY(c->work.reserve(mp::exact_scratch_bytes(g.n, g.P, g.W)));
+    c->work.used = 0;
+    return selfjoin_f64_run(c, c->work.base, d_ts, g, d_profile, d_index, o, tm);
+}
+
 int mp_selfjoin_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double* d_profile,
                         int64_t* d_index, const mp_opts* o, mp_timing* tm) {
     if (!c || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
@@ -329,6 +347,26 @@ int mp_selfjoin_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, dou
     std::lock_guard<std::mutex> guard(c->lock);
     MP_HIP(hipSetDevice(c->device));
     return selfjoin_f64_locked(c, d_ts, g, d_profile, d_index, o, tm);
+}
+
+// The exact self-join in the library's file-scope scratch array: no device
+// memory is allocated or freed.  Calls on one device share the array and are
+// serialised by its lock, on top of the context's.
+int mp_selfjoin_f64_dev_noalloc(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double* d_profile,
+                                int64_t* d_index, const mp_opts* o, mp_timing* tm) {
+    if (!c || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
+    Geometry g;
+    MP_TRY(geometry(n, m, o, &g));
+    if (n > mp::kNoAllocMaxPoints) return MP_ERR_TOO_LONG;
+    const size_t need = mp::exact_scratch_bytes(g.n, g.P, g.W);
+    if (need > mp::kNoAllocScratchBytes) return MP_ERR_TOO_LONG;
+    char* scratch = nullptr;
+    std::mutex* use = nullptr;
+    MP_TRY(mp::noalloc_scratch(c->device, &scratch, &use));
+    std::lock_guard<std::mutex> guard(c->lock);
+    std::lock_guard<std::mutex> guard2(*use);
+    MP_HIP(hipSetDevice(c->device));
+    return selfjoin_f64_run(c, scratch, d_ts, g, d_profile, d_index, o, tm);
 }
 
 // Host-pointer wrapper shared by the self-joins and PreSCRIMP.  The default

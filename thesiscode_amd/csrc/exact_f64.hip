@@ -640,6 +640,47 @@ __global__ __launch_bounds__(kGroup) void k_walk_f64_wide(MP_WALK_PARAMS) {
 }  // namespace mp
 
 // ---------------------------------------------------------------------------
+// Scratch of the allocation-free exact self-join.  The array's name is a host
+// stand-in in host code: kernels get the device address that
+// hipGetSymbolAddress returns for each device (kept in g_noalloc), never the
+// name itself.
+// ---------------------------------------------------------------------------
+namespace mp {
+
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct NoAllocSlot {
+    char* ptr = nullptr;
+    std::mutex use;  // held by a call while it works in the array
+};
+std::mutex g_noalloc_init;
+NoAllocSlot g_noalloc[kMaxDevices];
+}  // namespace
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return MP_ERR_BAD_ARG;
+    std::lock_guard<std::mutex> g(g_noalloc_init);
+    if (g_noalloc[device].ptr) return MP_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MP_ERR_DEVICE;
+    g_noalloc[device].ptr = static_cast<char*>(p);
+    return MP_OK;
+}
+
+int noalloc_scratch(int device, char** ptr, std::mutex** lock) {
+    if (device < 0 || device >= kMaxDevices) return MP_ERR_BAD_ARG;
+    std::lock_guard<std::mutex> g(g_noalloc_init);
+    if (!g_noalloc[device].ptr) return MP_ERR_DEVICE;
+    *ptr = g_noalloc[device].ptr;
+    *lock = &g_noalloc[device].use;
+    return MP_OK;
+}
+
+}  // namespace mp
+
+// ---------------------------------------------------------------------------
 // Host-side launchers (used by api.cpp).  All take device pointers and a stream.
 // ---------------------------------------------------------------------------
 namespace mp {

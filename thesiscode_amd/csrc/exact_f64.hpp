@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstddef>
+#include <mutex>
+
 namespace mp {
 
 // One window's record as the band walker reads it: A[i], A[i+m], 1/sigma and
@@ -14,6 +17,32 @@ struct alignas(32) WalkRec {
 };
 // padding records needed past the last row: one band of the widest walker
 constexpr int kWalkRecPad = 64 * 8 + 64;
+
+// Device bytes the exact self-join carves from its scratch for n points, P
+// windows and W walked diagonals (each buffer 256-byte aligned): A (n+64),
+// prefix sums (2n), 1/sigma and mu*sqrt(m)/sigma (P+64 each), seeds (W+1),
+// window records (P + kWalkRecPad), result values and keys (P each), counter.
+constexpr size_t exact_aligned(int64_t count, size_t elem) {
+    return (static_cast<size_t>(count) * elem + 255) & ~size_t(255);
+}
+constexpr size_t exact_scratch_bytes(int64_t n, int64_t P, int64_t W) {
+    return exact_aligned(n + 64, 8) + 2 * exact_aligned(n, 8) + 2 * exact_aligned(P + 64, 8) +
+           exact_aligned(W + 1, 8) + exact_aligned(P + kWalkRecPad, sizeof(WalkRec)) +
+           2 * exact_aligned(P, 8) + exact_aligned(1, 4);
+}
+
+// The allocation-free exact self-join works in one file-scope device array of
+// this library, sized for series of up to kNoAllocMaxPoints points (2^19, the
+// longest request of the serving workload): about 50 MB that every GPU the
+// library loads on holds until the process ends.
+constexpr int64_t kNoAllocMaxPoints = int64_t(1) << 19;
+constexpr size_t kNoAllocScratchBytes = exact_scratch_bytes(kNoAllocMaxPoints, kNoAllocMaxPoints, kNoAllocMaxPoints);
+// Resolve the array's address on `device` (which must be current), once per
+// device; later calls return the kept address.  MP_OK or MP_ERR_DEVICE.
+int noalloc_scratch_init(int device);
+// The array's device address on `device` (after noalloc_scratch_init) and the
+// lock that serialises the calls using it.
+int noalloc_scratch(int device, char** ptr, std::mutex** lock);
 
 int launch_ref_stats(const double* A, int n, int m, double* cs, double* sq, double* sInv,
                      double* mus, hipStream_t st);
