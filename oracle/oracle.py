@@ -41,6 +41,14 @@ def lib() -> C.CDLL:
         h.orc_window_moments_ld.argtypes = [_D, C.c_long, C.c_int, _D, _D]
         h.orc_set_fault.argtypes = [C.c_int]
         h.orc_prescrimp.argtypes = [_D, C.c_long, C.c_int, C.c_double, _L, C.c_long, _D, _L]
+        h.orc_partition_coords.argtypes = [C.c_long, _L, _L, C.POINTER(C.c_int)]
+        h.orc_distrib_geometry.argtypes = [C.c_long, C.c_int, C.c_int, _L, _L]
+        h.orc_distrib_geometry.restype = C.c_int
+        h.orc_distrib_tile_inputs.argtypes = [_D, C.c_long, C.c_int, C.c_int, C.c_long, _D, _D, _D, _D, _D, _D,
+                                              _D, _L, _L, C.POINTER(C.c_int)]
+        h.orc_distrib_tile_inputs.restype = C.c_int
+        h.orc_distrib_par.argtypes = [_D, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, _D, _L]
+        h.orc_distrib_par.restype = C.c_int
         _lib = h
     return _lib
 
@@ -161,3 +169,48 @@ class fault:
 
     def __exit__(self, *exc):
         lib().orc_set_fault(0)
+
+
+def partition_coords(rank: int):
+    """(row, col, is_upper) of a rank in the checkerboard (checkerboard_partitioning.cpp:8-41)."""
+    r, c = C.c_long(), C.c_long()
+    up = C.c_int()
+    lib().orc_partition_coords(rank, C.byref(r), C.byref(c), C.byref(up))
+    return r.value, c.value, bool(up.value)
+
+
+def distrib_geometry(n: int, m: int, k: int):
+    """(padding, tile length L) of distrib_par with k x k tiles; ValueError when it throws."""
+    pad, L = C.c_long(), C.c_long()
+    if lib().orc_distrib_geometry(n, m, k, C.byref(pad), C.byref(L)) != 0:
+        raise ValueError("parameters chosen badly")
+    return pad.value, L.value
+
+
+def distrib_tile_inputs(A, m: int, k: int, rank: int) -> dict:
+    """One tile's kernel inputs as distrib_par builds them: slices, their statistics, seeds."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    _, L = distrib_geometry(A.size, m, k)
+    out = {name: np.empty(L + m) for name in ("A", "B")}
+    for name in ("sA", "muA", "sB", "muB", "dots"):
+        out[name] = np.empty(L)
+    ro, co = C.c_long(), C.c_long()
+    up = C.c_int()
+    rc = lib().orc_distrib_tile_inputs(_d(A), A.size, m, k, rank, _d(out["A"]), _d(out["B"]), _d(out["sA"]),
+                                       _d(out["muA"]), _d(out["sB"]), _d(out["muB"]), _d(out["dots"]),
+                                       C.byref(ro), C.byref(co), C.byref(up))
+    if rc != 0:
+        raise ValueError("bad tile")
+    out.update(L=L, row_offset=ro.value, col_offset=co.value, upper=bool(up.value))
+    return out
+
+
+def distrib_par(A, m: int, k: int, *, blocklen: int = 500, kernel: int = 1):
+    """distrib_par with k x k tiles on one process; returns (profile, index) of the n - m + 1 entries."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    P = A.size - m + 1
+    prof = np.empty(max(P, 0))
+    idx = np.empty(max(P, 0), dtype=np.int64)
+    if lib().orc_distrib_par(_d(A), A.size, m, k, blocklen, kernel, _d(prof), _l(idx)) != 0:
+        raise ValueError("parameters chosen badly")
+    return prof, idx

@@ -547,3 +547,162 @@ void orc_prescrimp(const double* A, long n, int m, double stride, const long* or
     free(mean);
     free(sig);
 }
+
+/* ---- distrib_par: the k x k checkerboard of triangular tiles ---------------- */
+
+/* checkerboard_partitioning.cpp:8-41: row r holds ranks r^2 .. r^2+2r; column
+ * (rank - r^2)/2; upper when the rank's parity differs from the diagonal's. */
+void orc_partition_coords(long rank, long* row, long* col, int* is_upper) {
+    long r = 0, first = 0, next = 1;
+    while (next <= rank) {
+        r += 1;
+        first = r * r;
+        next = first + 2 * r + 1;
+    }
+    *row = r;
+    *col = (rank - first) / 2;
+    *is_upper = (r % 2 == 0) ? (rank % 2 == 1) : (rank % 2 == 0);
+}
+
+/* ScrimpDistribPar.cpp:580-596: the series is padded so that the triangle's
+ * base W = n + pad - m - ez splits into k tiles of L diagonals. */
+int orc_distrib_geometry(long n, int m, int k, long* pad, long* L) {
+    const long ez = m / 4;
+    if (k < 1 || m < 1) return -1;
+    const long work = n - m - ez;
+    const long floored = work / k;
+    const long delta = work - k * floored;
+    const long p = (delta > 0) ? k - delta : 0;
+    const long PL = n + p - m + 1;
+    const long W = PL - ez - 1;
+    if (PL <= 0 || W < k) return -1; /* ScrimpDistribPar.cpp:605-606 */
+    *pad = p;
+    *L = W / k;
+    return 0;
+}
+
+/* slice[x] = series[off + x] inside the series, 0 past its end
+ * (ScrimpDistribPar.cpp:724-767 with DISTRIB_BCAST_INPUT, settings.h:34) */
+static void distrib_slice(const double* series, long n, long off, long len, double* out) {
+    for (long x = 0; x < len; ++x) out[x] = (off + x < n) ? series[off + x] : 0.0;
+}
+
+int orc_distrib_tile_inputs(const double* series, long n, int m, int k, long rank, double* A, double* B,
+                            double* sA, double* muA, double* sB, double* muB, double* dots, long* row_offset,
+                            long* col_offset, int* is_upper) {
+    long pad, L, row, col;
+    if (orc_distrib_geometry(n, m, k, &pad, &L) != 0 || rank < 0 || rank >= (long)k * k) return -1;
+    const long ez = m / 4;
+    orc_partition_coords(rank, &row, &col, is_upper);
+    *row_offset = row * L + ez + 1; /* ScrimpDistribPar.cpp:666-668 */
+    *col_offset = col * L;
+    const long slen = L + m - 1; /* ts_slice_len */
+    distrib_slice(series, n, *row_offset, slen, A);
+    distrib_slice(series, n, *col_offset, slen, B);
+    /* statistics of each slice on its own (ScrimpDistribPar.cpp:780-781) */
+    orc_window_stats(A, slen, m, sA, muA);
+    orc_window_stats(B, slen, m, sB, muB);
+    /* padding invalidated after the statistics (ScrimpDistribPar.cpp:102-108, 783-802) */
+    for (long x = slen - pad; x < slen; ++x) {
+        if (col == k - 1) {
+            B[x] = 0.0;
+            if (x < L) sB[x] = muB[x] = 0.0;
+        }
+        if (row == k - 1) {
+            A[x] = 0.0;
+            if (x < L) sA[x] = muA[x] = 0.0;
+        }
+    }
+    A[slen] = 0.0; /* the element pushed back for the kernel's last read (:805-806) */
+    B[slen] = 0.0;
+    /* init_dotproducts (ScrimpDistribPar.cpp:86-100, 810-815):
+     * dots[d] = sum_t X[t + d] * Y[t], an FMA chain under -ffast-math */
+    const int swap = *is_upper && g_fault != 7;
+    const double* X = swap ? B : A;
+    const double* Y = swap ? A : B;
+    for (long d = 0; d < L; ++d) {
+        double z = 0.0;
+        for (int t = 0; t < m; ++t) z = fma(X[t + d], Y[t], z);
+        dots[d] = z;
+    }
+    return 0;
+}
+
+/* ScrimpDistribPar::eval_tile_blocked (ScrimpDistribPar.cpp:205-247) */
+void orc_eval_tile_blocked(int kernel, double prof_colmin[], long idx_colmin[], double prof_rowmin[],
+                           long idx_rowmin[], double tmpQ[], long blocklen, long trianglen, long tilelen,
+                           const double A_hor[], const double A_vert[], long w, const double s_hor[],
+                           const double mu_hor[], const double s_vert[], const double mu_vert[], long baserow,
+                           long basecol) {
+    orc_kernel_fn kern = kernel ? orc_eval_block_abba : orc_eval_block_scalar;
+    for (long bo = 0; bo < tilelen; bo += blocklen) {
+        const long nd = (blocklen < tilelen - bo) ? blocklen : tilelen - bo;
+        kern(prof_colmin, idx_colmin, prof_rowmin + bo, idx_rowmin + bo, tmpQ + bo, nd, trianglen - bo, A_hor,
+             A_vert + bo, w, s_hor, mu_hor, s_vert + bo, mu_vert + bo, baserow + bo, basecol);
+    }
+}
+
+int orc_distrib_par(const double* series, long n, int m, int k, int blocklen, int kernel, double* profile,
+                    long* index) {
+    long pad, L;
+    if (orc_distrib_geometry(n, m, k, &pad, &L) != 0) return -1;
+    const long ez = m / 4;
+    const long PL = n + pad - m + 1, P = n - m + 1;
+    const long slen = L + m - 1;
+    double* score = (double*)malloc(sizeof(double) * PL);
+    long* sidx = (long*)malloc(sizeof(long) * PL);
+    double* buf = (double*)malloc(sizeof(double) * (2 * (slen + 1) + 5 * L + 2 * (L + ez + 1)));
+    long* ibuf = (long*)malloc(sizeof(long) * (L + (L + ez + 1)));
+    double *A = buf, *B = A + slen + 1, *sA = B + slen + 1, *muA = sA + L, *sB = muA + L, *muB = sB + L;
+    double *dots = muB + L, *hor = dots + L, *vert = hor + L + ez + 1;
+    long *hidx = ibuf, *vidx = ibuf + L + ez + 1;
+    for (long g = 0; g < PL; ++g) {
+        score[g] = -1.0;
+        sidx[g] = -1;
+    }
+    for (long rank = 0; rank < (long)k * k; ++rank) {
+        long ro, co;
+        int up;
+        orc_distrib_tile_inputs(series, n, m, k, rank, A, B, sA, muA, sB, muB, dots, &ro, &co, &up);
+        for (long x = 0; x < L + ez + 1; ++x) {
+            hor[x] = -1.0;
+            hidx[x] = -1;
+        }
+        for (long x = 0; x < L; ++x) {
+            vert[x] = -1.0;
+            vidx[x] = -1;
+        }
+        if (up && g_fault == 6) continue;
+        if (up) /* the upper triangle, transposed (ScrimpDistribPar.cpp:819-839) */
+            orc_eval_tile_blocked(kernel, vert, vidx, hor + 1, hidx + 1, dots + 1, blocklen, L - 1, L - 1, A, B + 1,
+                                  m, sA, muA, sB + 1, muB + 1, co + 1, ro);
+        else /* ScrimpDistribPar.cpp:840-862 */
+            orc_eval_tile_blocked(kernel, hor, hidx, vert, vidx, dots, blocklen, L, L, B, A, m, sB, muB, sA, muA, ro,
+                                  co);
+        /* score-max merge (merge_score_profiles, ScrimpDistribPar.cpp:175-200): the
+         * horizontal result covers profile entries co + x, the vertical one ro + x */
+        for (long x = 0; x < L; ++x) {
+            if (hor[x] > score[co + x]) {
+                score[co + x] = hor[x];
+                sidx[co + x] = hidx[x];
+            }
+        }
+        for (long x = 0; x < L; ++x) {
+            if (vert[x] > score[ro + x]) {
+                score[ro + x] = vert[x];
+                sidx[ro + x] = vidx[x];
+            }
+        }
+    }
+    /* d2 = 2m - 2*score (ScrimpDistribPar.cpp:1010-1015); padding entries are not written */
+    const double twice_m = 2.0 * (double)m;
+    for (long g = 0; g < P; ++g) {
+        profile[g] = twice_m - 2.0 * score[g];
+        index[g] = sidx[g];
+    }
+    free(score);
+    free(sidx);
+    free(buf);
+    free(ibuf);
+    return 0;
+}

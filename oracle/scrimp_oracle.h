@@ -85,6 +85,42 @@ int orc_vert_blocked(const double* A, long n, int m, long excl, int blocklen, in
                      double* profile, long* index);
 
 /*
+ * distrib_par (ScrimpDistribPar.cpp:444-1116) on one process: the k x k
+ * checkerboard of tiles of k^2 ranks, restated without MPI.
+ *   geometry (:580-596): W = n + pad - m - ez with pad making W a multiple of
+ *     k, tile length L = W/k; rank -> (row, col, upper) by
+ *     get_partition_coords (checkerboard_partitioning.cpp:8-41); row offset
+ *     row*L + ez + 1, column offset col*L (:666-668).
+ *   tile inputs: slices of L + m - 1 points (0 past the series) at both
+ *     offsets, window statistics of each slice on its own (:780-781, the
+ *     ScrimpSequOpt arithmetic), padding of the last row/column zeroed after
+ *     them (:783-802), one 0.0 after each slice (:805-806), seeds
+ *     dots[d] = sum_t X[t+d]*Y[t] with (X, Y) = (A, B) for a lower tile and
+ *     (B, A) for an upper one (:86-100, 810-815).
+ *   evaluation: eval_tile_blocked (:205-247) on the lower triangle, or on the
+ *     upper triangle transposed (:819-862).
+ *   merge: every tile's horizontal result (entries col*L + x) and vertical
+ *     result (entries row*L + ez + 1 + x) into the profile with the score-max
+ *     rule of merge_score_profiles (:175-200), in rank order; then
+ *     d2 = 2m - 2*score (:1010-1015) for the n - m + 1 entries of the input.
+ * The reference reduces over row/column communicators (:899-1004); every step
+ * is the same strict maximum, so only exact ties can depend on the order.
+ */
+void orc_partition_coords(long rank, long* row, long* col, int* is_upper);
+int orc_distrib_geometry(long n, int m, int k, long* pad, long* L);
+/* A, B: L + m entries; sA, muA, sB, muB, dots: L entries */
+int orc_distrib_tile_inputs(const double* series, long n, int m, int k, long rank, double* A, double* B,
+                            double* sA, double* muA, double* sB, double* muB, double* dots, long* row_offset,
+                            long* col_offset, int* is_upper);
+void orc_eval_tile_blocked(int kernel, double prof_colmin[], long idx_colmin[], double prof_rowmin[],
+                           long idx_rowmin[], double tmpQ[], long blocklen, long trianglen, long tilelen,
+                           const double A_hor[], const double A_vert[], long w, const double s_hor[],
+                           const double mu_hor[], const double s_vert[], const double mu_vert[], long baserow,
+                           long basecol);
+int orc_distrib_par(const double* series, long n, int m, int k, int blocklen, int kernel, double* profile,
+                    long* index);
+
+/*
  * Brute force in long double: for every pair |i-j| > excl,
  * d2 = 2m(1 - rho_ij) with rho from two-pass centred sums; profile[i] =
  * min over j, index[i] = the first j attaining it.  O(P^2 m).
@@ -129,7 +165,8 @@ void orc_prescrimp(const double* A, long n, int m, double stride, const long* or
  * 1 = drop the A[i]*A[j] term of the recurrence, 2 = flip the sign of the
  * mean product in the score, 3 = report the row index for column results,
  * 4 = exclusion zone one diagonal too narrow, 5 = sample instead of
- * population variance.
+ * population variance, 6 = distrib_par skips its upper tiles, 7 = distrib_par
+ * seeds an upper tile without transposing it.
  */
 void orc_set_fault(int code);
 
