@@ -274,7 +274,8 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
     auto* val = cv.take<unsigned long long>(P);
     auto* key = cv.take<unsigned long long>(P);
     auto* counter = cv.take<unsigned int>(1);
-    const int ib = bits_for(P + 1);
+    // key codes: partners (band width 1), or a partner and a direction bit
+    const int ib = band_k > 1 ? bits_for(2 * P + 2) : bits_for(P + 1);
 
     MP_HIP(hipEventRecord(c->ev[0], st));
     MP_HIP(hipMemcpyAsync(A, d_ts, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
@@ -309,12 +310,14 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
         a.band_k = band_k;
         const int nb = mp::walk_f64_nbands(W, band_k);
         const int items = (nb + 1) / 2;
-        a.groups = std::max(1, std::min((items + 3) / 4, 8 * c->cus));
+        // two waves per SIMD when there are items for both (4 SIMDs per CU)
+        a.pair = (band_k == 8 && items >= 8 * c->cus) ? 1 : 0;
+        a.groups = std::max(1, std::min((items + 3) / 4, (a.pair ? 2 : 8) * c->cus));
         MP_TRY(mp::launch_walk_f64(a, st));
     }
     MP_HIP(hipEventRecord(c->ev[3], st));
-    MP_TRY(mp::launch_unpack_f64(val, key, static_cast<int>(P), m, ib, index_init, d_profile,
-                                 d_index, st));
+    MP_TRY(mp::launch_unpack_f64(val, key, static_cast<int>(P), m, ib, index_init, band_k, A, sInv, mus,
+                                 g.base, d_profile, d_index, st));
     MP_HIP(hipEventRecord(c->ev[4], st));
     MP_HIP(hipEventSynchronize(c->ev[4]));
     if (tm) {
@@ -457,7 +460,8 @@ static int prescrimp_locked(mp_ctx* c, const double* d_ts, const Geometry& g, do
     MP_TRY(mp::launch_prescrimp(a, st));
     MP_HIP(hipEventRecord(c->ev[3], st));
     // untouched entries keep d2 = +inf and index 0, as the reference initialises them
-    MP_TRY(mp::launch_unpack_f64(val, key, static_cast<int>(P), m, ib, 0, d_profile, d_index, st));
+    MP_TRY(mp::launch_unpack_f64(val, key, static_cast<int>(P), m, ib, 0, 1, nullptr, nullptr, nullptr, 0,
+                                 d_profile, d_index, st));
     MP_HIP(hipEventRecord(c->ev[4], st));
     MP_HIP(hipEventSynchronize(c->ev[4]));
     if (tm) {

@@ -21,6 +21,16 @@
 // reduces the 64 partials per column.  Row and column results are merged into
 // HBM with two 64-bit atomicMax per result: one on the exact ordered score,
 // one on a key of (truncated score, partner index).
+//
+// Lazy argmax.  A step keeps maxima only: the column maximum of a lane's K
+// cells and, per row, the maximum over the lane's cells of that row (the
+// exact scores, so the profile values stay bit-identical).  Which lane holds
+// the argmax is tracked exactly (a row carries the lane that last raised it;
+// a column's flush reduction keeps the lane of its maximum); which of that
+// lane's K diagonals it was is left open: a result's key holds the lane's
+// first candidate partner and the direction of the others, and the unpack
+// recomputes the K candidates' scores of each profile entry's winning key from
+// direct FP64 dot products and takes the one closest to the exact maximum.
 #include <type_traits>
 #include <utility>
 
@@ -134,16 +144,78 @@ __global__ void k_fill_u32(unsigned int* __restrict__ p, int count, unsigned int
     if (i < count) p[i] = v;
 }
 
+// score -> d^2 = 2m - 2*score and the key's partner.  Keys of a walker of
+// band width K > 1 name the K candidate partners of the winning lane's
+// diagonals (code = p0 << 1 | dir: p0 + k, or p0 - k when dir = 1); the one
+// whose recomputed score is closest to the exact maximum is the index (equal
+// distances: the smaller partner).  The candidates' windows overlap, so each
+// block of K window positions loads K values of window i and 2K-1 of the
+// candidates' and does K*K FMAs.  Candidates outside the profile or inside the
+// exclusion zone are not cells and are skipped.
+template <int K>
+__device__ __forceinline__ int64_t pick_candidate(const double* __restrict__ A, const double* __restrict__ sInv,
+                                                  const double* __restrict__ mus, int i, int64_t code, int P, int m,
+                                                  int base, double score) {
+    const int64_t p0 = code >> 1;
+    const int64_t lo = (code & 1) ? p0 - (K - 1) : p0;  // the candidates are lo .. lo+K-1
+    const int64_t ws = lo < 0 ? 0 : lo;                 // first window read (>= 0)
+    const double* __restrict__ x = A + i;
+    const double* __restrict__ y = A + ws;
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    int t = 0;
+    for (; t + K <= m; t += K) {
+        double xs[K], ys[2 * K - 1];
+#pragma unroll
+        for (int u = 0; u < K; ++u) xs[u] = x[t + u];
+#pragma unroll
+        for (int v = 0; v < 2 * K - 1; ++v) ys[v] = y[t + v];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+#pragma unroll
+            for (int u = 0; u < K; ++u) acc[k] = fma(xs[u], ys[u + k], acc[k]);
+        }
+    }
+    for (; t < m; ++t) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = fma(x[t], y[t + k], acc[k]);
+    }
+    int64_t best = -1;
+    double bestd = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int64_t p = ws + k;
+        const int64_t gap = p > i ? p - i : i - p;
+        if (p < lo || p >= lo + K || p >= P || gap < base) continue;
+        double dd = fabs(fma(acc[k], sInv[i] * sInv[p], -(mus[i] * mus[p])) - score);
+        if (!(dd == dd)) dd = __builtin_inf();
+        if (best < 0 || dd < bestd) {  // ascending p: an equal distance keeps the smaller partner
+            best = p;
+            bestd = dd;
+        }
+    }
+    return best >= 0 ? best : p0;
+}
+
+template <int K>
 __global__ void k_unpack_f64(const unsigned long long* __restrict__ val,
                              const unsigned long long* __restrict__ key, int P, int m, int ib,
-                             int64_t index_init, double* __restrict__ profile,
-                             int64_t* __restrict__ index) {
+                             int64_t index_init, const double* __restrict__ A,
+                             const double* __restrict__ sInv, const double* __restrict__ mus, int base,
+                             double* __restrict__ profile, int64_t* __restrict__ index) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     const double score = unord_f64(val[i]);
     profile[i] = 2.0 * static_cast<double>(m) - 2.0 * score;
-    const int64_t p = key_partner(key[i], ib);
-    index[i] = (p < 0) ? index_init : p;
+    const int64_t code = key_partner(key[i], ib);
+    if (code < 0) {
+        index[i] = index_init;
+    } else if constexpr (K == 1) {
+        index[i] = code;
+    } else {
+        index[i] = pick_candidate<K>(A, sInv, mus, i, code, P, m, base, score);
+    }
 }
 
 // Block drop-in unpack: decode a device range back into score-domain values;
@@ -189,17 +261,17 @@ __global__ void k_pack_rec(const double* __restrict__ A, const double* __restric
     out[i] = r;
 }
 
-// (value, local diagonal id) merge: the larger score wins; exact ties keep the
-// larger id so that merging is symmetric and deterministic.  NaN (a masked
-// cell) ranks below every number, so a NaN operand never absorbs a score and
-// the merge is a maximum over a total order: any reduction order gives the
-// same result.
+// (value, lane) merge: the larger score wins; exact ties keep the smaller lane
+// (the smaller diagonal, so the smaller partner, as the result keys rank exact
+// ties).  NaN (a masked cell) ranks below every number, so a NaN operand never
+// absorbs a score and the merge is a maximum over a total order: any reduction
+// order gives the same result.
 struct ScoreId {
     double v;
     int id;
 };
 __device__ __forceinline__ ScoreId better(ScoreId a, ScoreId b) {
-    const bool take = b.v > a.v || (a.v != a.v) || (b.v == a.v && b.id > a.id);
+    const bool take = b.v > a.v || (a.v != a.v) || (b.v == a.v && b.id < a.id);
     return take ? b : a;
 }
 // Maximum of two scores with the plain instruction: it returns the other
@@ -210,12 +282,6 @@ __device__ __forceinline__ double dmax(double a, double b) {
     double r;
     asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
     return r;
-}
-// Per-step merge of cells in diagonal order: the larger score wins, the left
-// (smaller diagonal) operand on ties, and a NaN loses to any number.
-__device__ __forceinline__ ScoreId merge_left(ScoreId a, ScoreId b) {
-    const double v = dmax(a.v, b.v);
-    return ScoreId{v, (a.v == v) ? a.id : b.id};
 }
 
 // better() over lanes l, l^16, l^32 and l^48, in every lane.  Each gfx950 row
@@ -233,19 +299,15 @@ __device__ __forceinline__ ScoreId better_quarters(ScoreId x) {
     return pair(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
 }
 
-template <int K, int N>
-struct ColTree {  // maximum of N consecutive cells starting at cell K0 (pairwise tree)
-    template <int K0>
-    __device__ __forceinline__ static ScoreId reduce(const double* sc, int lid0) {
-        if constexpr (N == 1) {
-            return ScoreId{sc[K0], lid0 + K0};
-        } else {
-            const ScoreId a = ColTree<K, N / 2>::template reduce<K0>(sc, lid0);
-            const ScoreId b = ColTree<K, N - N / 2>::template reduce<K0 + N / 2>(sc, lid0);
-            return merge_left(a, b);
-        }
+// Maximum of N consecutive cells starting at cell K0 (pairwise tree).
+template <int N, int K0 = 0>
+__device__ __forceinline__ double max_tree(const double* sc) {
+    if constexpr (N == 1) {
+        return sc[K0];
+    } else {
+        return dmax(max_tree<N / 2, K0>(sc), max_tree<N - N / 2, K0 + N / 2>(sc));
     }
-};
+}
 
 // WRITEBACK: store each diagonal's final QT (the block drop-in's tmpQ); the
 // self-join does not need it and skips the per-cell end-of-diagonal test.
@@ -274,7 +336,6 @@ __device__ __forceinline__ void walk_body(
     // the 16 lanes of an LDS phase then fall on distinct banks.  54,528 bytes
     // of LDS per group, so that 3 groups still fit in a CU's 160 KiB.
     __shared__ double s_cv[kWavesPerGroup][F][kWave + 1];
-    __shared__ int s_ci[kWavesPerGroup][F][kWave + 1];
     // [0, F): rows leaving lane 0; lane l > 0 writes scratch slot F + l + step
     __shared__ double s_rv[kWavesPerGroup][2 * F + kWave];
     __shared__ int s_ri[kWavesPerGroup][2 * F + kWave];
@@ -308,12 +369,11 @@ __device__ __forceinline__ void walk_body(
     const int nitems = (nbands + 1) / 2;
     const int lid0 = lane * K;
     double* const cv_out = &s_cv[wv][0][lane];
-    int* const ci_out = &s_ci[wv][0][lane];
     double* const rv_out = &s_rv[wv][(lane == 0) ? 0 : F + lane];
     int* const ri_out = &s_ri[wv][(lane == 0) ? 0 : F + lane];
 
     // Flush the results of columns [jstart, jstart+cnt) and of the rows that
-    // left lane 0 at those columns.  Row ids are stored as id+1 (0 = none).
+    // left lane 0 at those columns.  Row lanes are stored as lane+1 (0 = none).
     // pc/pr: the current results of this window's columns and rows, read one
     // window ahead.  Results only grow, so a plain (possibly stale) read filters
     // out the partials that cannot win; a stale read only lets an unneeded
@@ -324,16 +384,16 @@ __device__ __forceinline__ void walk_body(
     auto flush = [&](int jstart, int cnt, int dbase, uint64_t& pc, uint64_t& pr, auto reload)
         __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
-        // Lane (cs, qq) holds a quarter of column cs's 64 partials.  A column
-        // result passes the filter iff its score reaches thr, the score of the
-        // truncated current value (the order map is monotone), so a column
-        // can pass only if one of its partials does.  The reduction (a
-        // pairwise tree, then the other quarters; better() is a maximum over
-        // (score, id), so the order is free) runs only when some partial of
-        // the window passes.  For cs >= cnt the quarter is stale and unused.
+        // Lane (cs, qq) holds a quarter of column cs's 64 partials (those of
+        // lanes 16 qq .. 16 qq + 15).  A column result passes the filter iff
+        // its score reaches thr, the score of the truncated current value (the
+        // order map is monotone), so a column can pass only if one of its
+        // partials does.  The reduction (a pairwise tree, then the other
+        // quarters; better() is a maximum over (score, lane), so the order is
+        // free) runs only when some partial of the window passes.  For
+        // cs >= cnt the quarter is stale and unused.
         const int cs = lane & (F - 1), qq = lane / F;
         const double* const pv = &s_cv[wv][cs][qq * (kWave / 4)];
-        const int* const pi = &s_ci[wv][cs][qq * (kWave / 4)];
         double v[kWave / 4];
 #pragma unroll
         for (int u = 0; u < kWave / 4; ++u) v[u] = pv[u];
@@ -353,7 +413,7 @@ __device__ __forceinline__ void walk_body(
         if (__builtin_amdgcn_ballot_w64(cand) != 0) {
             ScoreId c[kWave / 4];
 #pragma unroll
-            for (int u = 0; u < kWave / 4; ++u) c[u] = ScoreId{v[u], pi[u]};
+            for (int u = 0; u < kWave / 4; ++u) c[u] = ScoreId{v[u], qq * (kWave / 4) + u};
 #pragma unroll
             for (int w = kWave / 8; w >= 1; w >>= 1) {
 #pragma unroll
@@ -366,21 +426,32 @@ __device__ __forceinline__ void walk_body(
         // a NaN result: every partial of the column was a masked cell
         const bool do_col = (lane < cnt) & (b.id >= 0) & (b.v == b.v) & ((oc >> ib) >= (pc >> ib));
         const double rvl = s_rv[wv][lane & (F - 1)];
-        const int rid = s_ri[wv][lane & (F - 1)] - 1;
+        const int rlane = s_ri[wv][lane & (F - 1)] - 1;
         const uint64_t orw = ord_f64(rvl);
-        const bool do_row = (lane < cnt) & (rid >= 0) & ((orw >> ib) >= (pr >> ib));
+        const bool do_row = (lane < cnt) & (rlane >= 0) & ((orw >> ib) >= (pr >> ib));
         __builtin_amdgcn_sched_barrier(0);  // tests first: the reload then reuses pc/pr
         reload(pc, pr);
+        // Key codes.  K = 1: the partner itself (the lane is the diagonal).
+        // K > 1: the lane's diagonals are candidates, and the code holds the
+        // partner of its first diagonal (p0) and the direction in which the
+        // candidates run: p0 + k for a column result (rows jj + d), p0 - k for
+        // a row result (columns jj - d); the unpack picks among them.
+        int64_t ccode, rcode;
+        if constexpr (K == 1) {
+            ccode = static_cast<int64_t>(jj) + dbase + b.id + col_off;
+            rcode = static_cast<int64_t>(jj) - rlane + row_off;
+        } else {
+            ccode = (static_cast<int64_t>(jj) + dbase + b.id * K + col_off) << 1;
+            rcode = ((static_cast<int64_t>(jj) - rlane * K + row_off) << 1) | 1;
+        }
         if (do_col) {
             atomicMax(&colval[jj], static_cast<unsigned long long>(oc));
-            atomicMax(&colkey[jj], static_cast<unsigned long long>(
-                                       make_key(oc, ib, static_cast<int64_t>(jj) + dbase + b.id + col_off)));
+            atomicMax(&colkey[jj], static_cast<unsigned long long>(make_key(oc, ib, ccode)));
         }
         if (do_row) {
             const int i = jj + dbase;
             atomicMax(&rowval[i], static_cast<unsigned long long>(orw));
-            atomicMax(&rowkey[i], static_cast<unsigned long long>(
-                                      make_key(orw, ib, static_cast<int64_t>(jj - rid) + row_off)));
+            atomicMax(&rowkey[i], static_cast<unsigned long long>(make_key(orw, ib, rcode)));
         }
         __builtin_amdgcn_wave_barrier();
     };
@@ -399,8 +470,11 @@ __device__ __forceinline__ void walk_body(
             // one is a padding record with sInv = NaN, a diagonal past the block
             // gets QT = NaN, and a NaN score never wins a comparison (nor does
             // it in the reference).
-            double q[K], wa[K], wam[K], ws[K], wm[K], rv[K];
-            int ri[K];
+            // per row slot: rin = the row's maximum over the lanes above this
+            // one (exact), tag = the lane that holds it + 1 (0: none), lm = the
+            // maximum over this lane's cells of the row so far
+            double q[K], wa[K], wam[K], ws[K], wm[K], rin[K], lm[K];
+            int tag[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int d = dbase + lid0 + k;
@@ -410,8 +484,9 @@ __device__ __forceinline__ void walk_body(
                 wam[k] = r.am;
                 ws[k] = r.s;
                 wm[k] = r.mu;
-                rv[k] = NEG;
-                ri[k] = 0;
+                rin[k] = NEG;
+                lm[k] = NEG;
+                tag[k] = 0;
                 if constexpr (RING) {  // rows 0 .. BW-1 of the band, slot = row - dbase
                     ring_put(lid0 + k, r);
                     if (lid0 + k < F) ring_put(RR + lid0 + k, r);
@@ -439,10 +514,9 @@ __device__ __forceinline__ void walk_body(
             // wait (lgkmcnt, which also counts LDS writes) then never waits for
             // writes issued just before it.
             double pcv = NEG, prv = NEG;
-            int pci = -1, pri = 0;
+            int pri = 0;
             auto put_partials = [&](int w) __attribute__((always_inline)) {
                 cv_out[w * (kWave + 1)] = pcv;
-                ci_out[w * (kWave + 1)] = pci;
                 rv_out[w] = prv;
                 ri_out[w] = pri;
             };
@@ -502,17 +576,19 @@ __device__ __forceinline__ void walk_body(
                         if (j == tl - 1 - (dbase + lid0 + k) && dbase + lid0 + k < blocklen)
                             qt[dbase + lid0 + k] = q[k];
                     }
-                    // the first (smallest-column) maximum wins; NaN never does
-                    ri[sl] = (sc[k] > rv[sl]) ? lid0 + k + 1 : ri[sl];
-                    rv[sl] = dmax(rv[sl], sc[k]);
+                    // the row in slot (S + K - 1) % K entered this lane at the end
+                    // of the last step: its first cell here starts lm
+                    lm[sl] = (k == K - 1) ? sc[k] : dmax(lm[sl], sc[k]);
                 }
-                const ScoreId col = ColTree<K, K>::template reduce<0>(sc, lid0);
-                pcv = col.v;
-                pci = col.id;
-                prv = rv[S];
-                pri = ri[S];
-                rv[S] = shift_down_f64(rv[S], NEG);
-                ri[S] = shift_down_i32_zero(ri[S]);
+                pcv = max_tree<K>(sc);
+                // the row in slot S leaves this lane: its maximum so far, and
+                // this lane as the argmax's when its own cells beat the lanes
+                // above (strictly: the row met those lanes first, at smaller
+                // columns); a NaN lm (only masked cells) changes nothing
+                prv = dmax(rin[S], lm[S]);
+                pri = (lm[S] > rin[S]) ? lane + 1 : tag[S];
+                rin[S] = shift_down_f64(prv, NEG);
+                tag[S] = shift_down_i32_zero(pri);
                 if constexpr (RING) {
                     wa[S] = nr.a;
                     wam[S] = nr.am;
@@ -634,6 +710,15 @@ template <int K, bool WRITEBACK>
 __global__ __launch_bounds__(kGroup) void k_walk_f64_wide(MP_WALK_PARAMS) {
     walk_body<K, WRITEBACK, true>(MP_WALK_ARGS);
 }
+// K = 8 with its rows shifted down the lanes by DPP instead of the LDS ring:
+// 38 KB of LDS per group and at most 256 VGPRs, so two waves share each SIMD
+// and hide each other's latency; it pays when there are work items for two
+// waves per SIMD.
+template <int K, bool WRITEBACK>
+__global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(2))) void k_walk_f64_pair(
+    MP_WALK_PARAMS) {
+    walk_body<K, WRITEBACK, false>(MP_WALK_ARGS);
+}
 #undef MP_WALK_PARAMS
 #undef MP_WALK_ARGS
 
@@ -715,10 +800,29 @@ int launch_fill_u32(unsigned int* p, int count, unsigned int v, hipStream_t st) 
 }
 
 int launch_unpack_f64(const unsigned long long* val, const unsigned long long* key, int P, int m,
-                      int ib, int64_t index_init, double* profile, int64_t* index,
-                      hipStream_t st) {
-    hipLaunchKernelGGL(k_unpack_f64, dim3((P + 255) / 256), dim3(256), 0, st, val, key, P, m, ib,
-                       index_init, profile, index);
+                      int ib, int64_t index_init, int band_k, const double* A, const double* sInv,
+                      const double* mus, int base, double* profile, int64_t* index, hipStream_t st) {
+    const dim3 grid((P + 255) / 256), block(256);
+    switch (band_k) {
+        case 1:
+            hipLaunchKernelGGL(k_unpack_f64<1>, grid, block, 0, st, val, key, P, m, ib, index_init, A, sInv, mus,
+                               base, profile, index);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_unpack_f64<2>, grid, block, 0, st, val, key, P, m, ib, index_init, A, sInv, mus,
+                               base, profile, index);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_unpack_f64<4>, grid, block, 0, st, val, key, P, m, ib, index_init, A, sInv, mus,
+                               base, profile, index);
+            break;
+        case 8:
+            hipLaunchKernelGGL(k_unpack_f64<8>, grid, block, 0, st, val, key, P, m, ib, index_init, A, sInv, mus,
+                               base, profile, index);
+            break;
+        default:
+            return MP_ERR_BAD_ARG;
+    }
     return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
 }
 
@@ -768,13 +872,17 @@ int launch_walk_f64(const WalkF64Launch& a, hipStream_t st) {
                                a.qt, a.rowval, a.rowkey, a.colval, a.colkey, a.counter,          \
                                a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib, nbands);       \
         break;
-    switch (a.band_k) {
-        MP_WALK_CASE(1, k_walk_f64)
-        MP_WALK_CASE(2, k_walk_f64)
-        MP_WALK_CASE(4, k_walk_f64)
-        MP_WALK_CASE(8, k_walk_f64_wide)
-        default:
-            return MP_ERR_BAD_ARG;
+    if (a.band_k == 8 && a.pair) {
+        switch (a.band_k) { MP_WALK_CASE(8, k_walk_f64_pair) }
+    } else {
+        switch (a.band_k) {
+            MP_WALK_CASE(1, k_walk_f64)
+            MP_WALK_CASE(2, k_walk_f64)
+            MP_WALK_CASE(4, k_walk_f64)
+            MP_WALK_CASE(8, k_walk_f64_wide)
+            default:
+                return MP_ERR_BAD_ARG;
+        }
     }
 #undef MP_WALK_CASE
     return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;

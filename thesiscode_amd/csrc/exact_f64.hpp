@@ -49,9 +49,12 @@ int launch_ref_stats(const double* A, int n, int m, double* cs, double* sq, doub
 int launch_ref_seeds(const double* A, int base, int count, int m, double* qt, hipStream_t st);
 int launch_fill_u64(unsigned long long* p, int64_t count, unsigned long long v, hipStream_t st);
 int launch_fill_u32(unsigned int* p, int count, unsigned int v, hipStream_t st);
+// profile/index from the merged (value, key) arrays.  band_k: the band width of
+// the walker whose keys these are (1: keys hold partners; 2, 4, 8: keys hold
+// candidate codes, resolved against A/sInv/mus, base = exclusion + 1).
 int launch_unpack_f64(const unsigned long long* val, const unsigned long long* key, int P, int m,
-                      int ib, int64_t index_init, double* profile, int64_t* index,
-                      hipStream_t st);
+                      int ib, int64_t index_init, int band_k, const double* A, const double* sInv,
+                      const double* mus, int base, double* profile, int64_t* index, hipStream_t st);
 int launch_pack_block(const double* prof, int count, int ib, unsigned long long* val,
                       unsigned long long* key, hipStream_t st);
 int launch_unpack_block(const unsigned long long* val, const unsigned long long* key, int count,
@@ -69,6 +72,7 @@ struct WalkF64Launch {
     unsigned int* counter;
     int64_t row_off, col_off;
     int blocklen, tl, m, ib, writeback, band_k, groups;
+    int pair;               // band_k = 8: two waves per SIMD, rows by DPP (k_walk_f64_pair)
 };
 int walk_f64_nbands(int blocklen, int band_k);
 int launch_walk_f64(const WalkF64Launch& a, hipStream_t st);
