@@ -133,11 +133,17 @@ def test_fp64_accumulation_removes_the_drift():
 def test_band_width_variants_agree(band_k, accum64):
     A = tc.random_walk(71, 30000)
     m = 64
+    # main tiles start at the same columns whatever the band width (edge tiles
+    # do not), so the FP32 maxima agree to the recurrence's rounding; the
+    # partner is the best (in FP64) of the K diagonals of the lane holding the
+    # maximum, so it may differ between band widths only among near ties
+    pu_ref, _, _ = tc.selfjoin_f32r(A, m, band_k=8, tile_len=2048, accum64=accum64, refine=False)
+    pu, _, _ = tc.selfjoin_f32r(A, m, band_k=band_k, tile_len=2048, accum64=accum64, refine=False)
+    assert np.all(rel_err(pu, pu_ref, m) <= 1e-4)
     p_ref, i_ref, _ = tc.selfjoin_f32r(A, m, band_k=8, tile_len=2048, accum64=accum64)
     p, i, _ = tc.selfjoin_f32r(A, m, band_k=band_k, tile_len=2048, accum64=accum64)
-    # tiles are seeded identically whatever the band width, so results match exactly
-    assert np.array_equal(i, i_ref)
-    assert np.array_equal(p, p_ref)
+    assert np.all(rel_err(p, p_ref, m) <= 1e-4)
+    assert np.mean(i == i_ref) >= 0.99
 
 
 @pytest.mark.parametrize("band_k", [4, 8, 16])

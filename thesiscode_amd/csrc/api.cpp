@@ -106,23 +106,40 @@ int geometry(int64_t n, int32_t m, const mp_opts* o, Geometry* g) {
 }
 
 // Band width of the exact walker.  A work item is a pair of bands of 64*K
-// diagonals, so all items hold the same W*64K cells and a SIMD with w items
-// takes w times as long: the time goes as ceil(items / SIMDs) * K * c_K, with
-// c_K the measured cost per cell of width K (relative, MI355X: the row shift
-// is spread over K cells).  Pick the K that minimises it.
-int pick_band_k_f64(const mp_ctx* c, int W, int requested) {
-    if (requested > 0) return requested;
-    static const int ks[4] = {1, 2, 4, 8};
-    static const double cost[4] = {134.0, 65.0, 37.0, 34.0};
-    const int64_t simds = 4LL * std::max(1, c->cus);
-    int best = 4;
+// diagonals, so all items hold the same W*64K cells.  With I items on S SIMDs
+// and at most `occ` waves of the kernel per SIMD, s = min(occ, ceil(I/S)) waves
+// share a SIMD and the walk takes ceil(I / (S*s)) rounds of one item each:
+// time ~ rounds * 64K * c(K, s), c the measured cost per cell of one wave of
+// width K while s waves share its SIMD (MI355X, ns per cell): one wave alone
+// 2.19, 1.30, 0.75 (K = 1, 2, 4) and 0.38 for K = 8 with its LDS row ring;
+// s waves run at s * g(s) times the single-wave cost per wave, g = 1, 0.69,
+// 0.57, 0.49 for s = 1..4; K = 8 with DPP rows, two waves per SIMD (used when
+// there are more items than SIMDs): 0.637 per wave.  Pick the K that
+// minimises it.
+struct WalkChoice {
+    int band_k;
+    int pair;  // K = 8 at two waves per SIMD
+};
+WalkChoice pick_band_k_f64(const mp_ctx* c, int W, int requested) {
+    const int64_t S = 4LL * std::max(1, c->cus);
+    auto items_of = [&](int k) { return (static_cast<int64_t>(mp::walk_f64_nbands(W, k)) + 1) / 2; };
+    if (requested > 0) return WalkChoice{requested, (requested == 8 && items_of(8) > S) ? 1 : 0};
+    static const double c1[4] = {2.19, 1.30, 0.75, 0.38};
+    static const double g[5] = {0.0, 1.0, 0.69, 0.57, 0.49};
+    WalkChoice best{4, 0};
     double best_t = 0.0;
     for (int i = 0; i < 4; ++i) {
-        const int64_t items = (mp::walk_f64_nbands(W, ks[i]) + 1) / 2;
-        const double t = static_cast<double>((items + simds - 1) / simds) * ks[i] * cost[i];
+        const int k = 1 << i;
+        const int64_t items = items_of(k);
+        const bool pair = (k == 8 && items > S);
+        const int occ = (k == 8) ? (pair ? 2 : 1) : 4;
+        const int64_t s = std::max<int64_t>(1, std::min<int64_t>(occ, (items + S - 1) / S));
+        const int64_t rounds = (items + S * s - 1) / (S * s);
+        const double per_wave = pair ? 0.637 : c1[i] * static_cast<double>(s) * g[s];
+        const double t = static_cast<double>(rounds) * 64.0 * k * per_wave;
         if (i == 0 || t < best_t) {
             best_t = t;
-            best = ks[i];
+            best = WalkChoice{k, pair ? 1 : 0};
         }
     }
     return best;
@@ -261,7 +278,8 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
     const int m = g.m;
     const int64_t index_init = (o && o->index_init == 0) ? 0 : -1;
     const int W = g.W;
-    const int band_k = pick_band_k_f64(c, W, o ? o->band_k : 0);
+    const WalkChoice wc = pick_band_k_f64(c, W, o ? o->band_k : 0);
+    const int band_k = wc.band_k;
     if (band_k != 1 && band_k != 2 && band_k != 4 && band_k != 8) return MP_ERR_BAD_ARG;
     Carve cv{scratch};
     double* A = cv.take<double>(n + 64);
@@ -310,8 +328,7 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
         a.band_k = band_k;
         const int nb = mp::walk_f64_nbands(W, band_k);
         const int items = (nb + 1) / 2;
-        // two waves per SIMD when there are items for both (4 SIMDs per CU)
-        a.pair = (band_k == 8 && items >= 8 * c->cus) ? 1 : 0;
+        a.pair = wc.pair;
         a.groups = std::max(1, std::min((items + 3) / 4, (a.pair ? 2 : 8) * c->cus));
         MP_TRY(mp::launch_walk_f64(a, st));
     }

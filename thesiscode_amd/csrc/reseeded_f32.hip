@@ -18,12 +18,15 @@
 // dg[i] = (A[i+m]-mu[i+1]) + (A[i]-mu[i]).  Results use the reference's
 // d^2 = 2m - 2*score with score = m*rho (ScrimpTrivialPar.cpp:336-340).
 //
-// Register trick: a cell's result is the double whose high word is the FP32
-// bits of corr+1 (>= 0) and whose low word is the cell's local diagonal id
-// K*lane+k.  Doubles with a non-negative high word order like that word and
-// then the low word, so a single v_max_f64 keeps the row (or column) maximum
-// together with the diagonal that produced it.  The partner follows from the
-// diagonal: column j = i - d for a row, row i = j + d for a column.
+// Lazy argmax.  A cell's result is c1 = corr + 1 (>= 0) in FP32, and a step
+// keeps 32-bit maxima only: the column maximum of a lane's K cells and, per
+// row, the maximum over the lane's cells of that row.  The lane that holds an
+// argmax is tracked exactly (a row carries the lane that last raised it; a
+// column's flush reduction keeps the lane of its maximum); which of the
+// lane's K diagonals it was is left open.  A result's key holds the lane's
+// first candidate partner and the direction of the others (p0 + k for a
+// column result, rows j + d; p0 - k for a row result, columns i - d), and the
+// unpack recomputes the K candidates' correlations in FP64 and keeps the best.
 #include <type_traits>
 #include <utility>
 
@@ -146,36 +149,53 @@ __global__ void k_init_keys(const SeriesDesc* __restrict__ descs,
                       0xFFFFFFFFull;
 }
 
-__device__ __forceinline__ unsigned long long global_key(double k, int64_t partner) {
-    float c1 = __int_as_float(__double2hiint(k));
+// Global key: FP32 bits of c1 above, 0xFFFFFFFE - code below (a smaller code
+// wins an exact tie; 0xFFFFFFFF is the initial "no partner" word).  code =
+// p0 << 1 | dir (dir = 1: the candidates run p0, p0 - 1, ...).
+__device__ __forceinline__ unsigned long long global_key(float c1, int64_t code) {
     const uint32_t bits = (c1 > 0.0f) ? __float_as_uint(c1) : 0u;
     return (static_cast<unsigned long long>(bits) << 32) |
-           static_cast<unsigned long long>(0xFFFFFFFEu - static_cast<uint32_t>(partner));
+           static_cast<unsigned long long>(0xFFFFFFFEu - static_cast<uint32_t>(code));
 }
 
-// Maximum of two tile keys.  A key's high word is the FP32 bit pattern of a
-// finite corr + bias, so a key is never a double NaN; the plain instruction
-// then needs none of the quieting that a generic fmax inserts per operand.
-__device__ __forceinline__ double key_max(double a, double b) {
-    double r;
-    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+// Maximum of two c1 values.  c1 is a finite FP32 number (never NaN), so the
+// plain instruction needs none of the quieting a generic fmaxf inserts.
+__device__ __forceinline__ float c1_max(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
 
-// Maximum of a key over lanes l, l^16, l^32 and l^48, in every lane.  Each
-// gfx950 row swap leaves a lane with its own dword and the dword of lane l^32
-// (l^16) in two registers, in some order; the maximum does not depend on it.
-__device__ __forceinline__ double key_max_quarters(double x) {
-    auto pair = [](double v, auto swap) __attribute__((always_inline)) {
-        const unsigned lo = static_cast<unsigned>(__double2loint(v));
-        const unsigned hi = static_cast<unsigned>(__double2hiint(v));
-        const auto l = swap(lo);
-        const auto h = swap(hi);
-        return key_max(__hiloint2double(static_cast<int>(h[0]), static_cast<int>(l[0])),
-                       __hiloint2double(static_cast<int>(h[1]), static_cast<int>(l[1])));
+// (c1, lane): the larger c1 wins, the smaller lane on exact ties.
+struct C1Lane {
+    float v;
+    int lane;
+};
+__device__ __forceinline__ C1Lane c1_better(C1Lane a, C1Lane b) {
+    const bool take = (b.v > a.v) | ((b.v == a.v) & (b.lane < a.lane));
+    return take ? b : a;
+}
+// c1_better() over lanes l, l^16, l^32 and l^48, in every lane.  Each gfx950
+// row swap leaves a lane with its own dword and the dword of lane l^32 (l^16)
+// in two registers, in some order; the result does not depend on it.
+__device__ __forceinline__ C1Lane c1_better_quarters(C1Lane x) {
+    auto pair = [](C1Lane v, auto swap) __attribute__((always_inline)) {
+        const auto a = swap(__float_as_uint(v.v));
+        const auto l = swap(static_cast<unsigned>(v.lane));
+        return c1_better(C1Lane{__uint_as_float(a[0]), static_cast<int>(l[0])},
+                         C1Lane{__uint_as_float(a[1]), static_cast<int>(l[1])});
     };
     x = pair(x, [](unsigned v) { return __builtin_amdgcn_permlane32_swap(v, v, false, false); });
     return pair(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
+}
+
+template <int N, int K0 = 0>
+__device__ __forceinline__ float c1_max_tree(const float* c) {
+    if constexpr (N == 1) {
+        return c[K0];
+    } else {
+        return c1_max(c1_max_tree<N / 2, K0>(c), c1_max_tree<N - N / 2, K0 + N / 2>(c));
+    }
 }
 
 // ACC64 = false: FP32 recurrence with FP32 accumulation.  ACC64 = true: each
@@ -203,14 +223,18 @@ __device__ __forceinline__ void tiles_body(
     constexpr int F = kFlush;
     constexpr int BW = kWave * K;
     static_assert(F % K == 0, "flush period must be a multiple of K");
-    static_assert(BW * sizeof(double) <= F * (kWave + 1) * sizeof(double), "seed staging fits");
-    __shared__ double s_ck[kWavesPerGroup][F][kWave + 1];  // column partials; seed staging
-    __shared__ double s_rk[kWavesPerGroup][2 * F + kWave]; // [0, F) row exits; the rest scratch
+    // column partials (F rows of 64, padded to 65) and, before a tile's steps,
+    // the staging of its FP64 seeds
+    constexpr int kBufDoubles = (BW > F * (kWave + 1) / 2) ? BW : F * (kWave + 1) / 2;
+    __shared__ double s_buf[kWavesPerGroup][kBufDoubles];
+    __shared__ float s_rk[kWavesPerGroup][2 * F + kWave];  // [0, F) row exits; the rest scratch
+    __shared__ int s_rl[kWavesPerGroup][2 * F + kWave];    // their lanes + 1 (0: none)
 
     const int wv = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     const int lid0 = lane * K;
-    double* seeds = &s_ck[wv][0][0];
+    double* seeds = &s_buf[wv][0];
+    float* const ck_base = reinterpret_cast<float*>(&s_buf[wv][0]);  // [F][kWave + 1]
 
     for (;;) {
         unsigned int tv = 0;
@@ -301,7 +325,11 @@ __device__ __forceinline__ void tiles_body(
         double covd[ACC64 ? K : 1];
         float chi[ACC64 ? K : 1];  // fl32(covd), refreshed at each fold
         float wiv[K];
-        double rk[K];
+        // per row slot: rin = the row's maximum over the lanes above this one,
+        // tag = the lane that holds it + 1 (0: none), lm = the maximum over
+        // this lane's cells of the row so far
+        float rin[K], lm[K];
+        int tag[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if constexpr (ACC64) {
@@ -315,13 +343,16 @@ __device__ __forceinline__ void tiles_body(
             wdf[k] = t.df;
             wdg[k] = t.dg;
             wiv[k] = t.inv;
-            rk[k] = 0.0;
+            rin[k] = -1.0f;  // below every c1 (>= 0)
+            lm[k] = -1.0f;
+            tag[k] = 0;
         }
         __builtin_amdgcn_wave_barrier();
         // lane 0 stores the row leaving the band at s_rk[wv][fslot]; lane l > 0
         // stores into s_rk[wv][F + l + fslot], a scratch slot nobody reads
-        double* const rk_out = &s_rk[wv][(lane == 0) ? 0 : F + lane];
-        double* const ck_out = &s_ck[wv][0][lane];
+        float* const rk_out = &s_rk[wv][(lane == 0) ? 0 : F + lane];
+        int* const rl_out = &s_rl[wv][(lane == 0) ? 0 : F + lane];
+        float* const ck_out = ck_base + lane;
 
         // The filter: a partial is merged only if it beats the key read when
         // the window started (lane x < F holds those of column c0 + x and of its
@@ -333,29 +364,33 @@ __device__ __forceinline__ void tiles_body(
             pc = keys[coff + jj];
             pr = keys[roff + jj + dbase];
         };
+        // Column results hold p0 = the partner of the lane's first diagonal
+        // (candidates p0 + k); row results p0 = the column of that diagonal
+        // (candidates p0 - k).
         auto flush = [&](int jstart, int cnt) __attribute__((always_inline)) {
             __builtin_amdgcn_wave_barrier();
-            // lane (cs, qq) reduces a quarter of column cs's 64 partials (for
-            // cs >= cnt a stale quarter, whose result is not used)
+            // lane (cs, qq) reduces a quarter of column cs's 64 partials, those
+            // of lanes 16 qq .. 16 qq + 15 (for cs >= cnt a stale quarter, whose
+            // result is not used)
             const int cs = lane & (F - 1), qq = lane / F;
-            const double* part = &s_ck[wv][cs][qq * (kWave / 4)];
-            double bk = part[0];
+            const float* part = ck_base + cs * (kWave + 1) + qq * (kWave / 4);
+            C1Lane b{part[0], qq * (kWave / 4)};
 #pragma unroll
-            for (int v = 1; v < kWave / 4; ++v) bk = key_max(bk, part[v]);
-            bk = key_max_quarters(bk);
-            if (lane < cnt && __double2hiint(bk) > 0) {
+            for (int v = 1; v < kWave / 4; ++v) b = c1_better(b, C1Lane{part[v], qq * (kWave / 4) + v});
+            b = c1_better_quarters(b);
+            if (lane < cnt && b.v > 0.0f) {
                 const int jj = jstart + lane;
-                const int lid = __double2loint(bk);
-                const int64_t partner = static_cast<int64_t>(d.base) + jj + dbase + lid;
-                const unsigned long long gk = global_key(bk, partner);
+                const int64_t p0 = static_cast<int64_t>(d.base) + jj + dbase + b.lane * K;
+                const unsigned long long gk = global_key(b.v, p0 << 1);
                 if (gk > pc) atomicMax(&keys[coff + jj], gk);
             }
             if (lane < cnt) {
-                const double rkv = s_rk[wv][lane];
-                if (__double2hiint(rkv) > 0) {
+                const float rkv = s_rk[wv][lane];
+                const int rl = s_rl[wv][lane] - 1;
+                if (rl >= 0 && rkv > 0.0f) {
                     const int jj = jstart + lane;
-                    const int lid = __double2loint(rkv);
-                    const unsigned long long gk = global_key(rkv, jj - lid);
+                    const int64_t p0 = static_cast<int64_t>(jj) - rl * K;
+                    const unsigned long long gk = global_key(rkv, (p0 << 1) | 1);
                     if (gk > pr) atomicMax(&keys[roff + jj + dbase], gk);
                 }
             }
@@ -364,10 +399,12 @@ __device__ __forceinline__ void tiles_body(
         // The row entering lane 63 at column j, clamped like the others.
         auto feed_row = [&](int j) __attribute__((always_inline)) { return min(j + dbase + BW, tl - 1); };
         Rec cur_c, cur_f;  // the next step's column and feed records
-        double pck = 0.0, prk = 0.0;  // the last step's column partial and row exit
+        float pck = -1.0f, prk = -1.0f;  // the last step's column partial and row exit
+        int prl = 0;
         auto put_partials = [&](int w) __attribute__((always_inline)) {
             ck_out[w * (kWave + 1)] = pck;
             rk_out[w] = prk;
+            rl_out[w] = prl;
         };
 
         // One column j = c0 + W of the band; W is the step's place in the flush
@@ -391,33 +428,32 @@ __device__ __forceinline__ void tiles_body(
             __builtin_amdgcn_sched_barrier(0);
             const float dfh = c.df, dgh = c.dg;
             const float ivh = c.inv;
-            double ck;
+            float c1[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int sl = (S + k) % K;
                 // the same three roundings in both forms: of the covariance to
                 // FP32, of its product with 1/norm_i, and of the final FMA
-                float c1;
                 if constexpr (ACC64) {
-                    c1 = fmaf(fmaf(chi[k], wiv[sl], cov[k] * wiv[sl]), ivh, 1.0f);
+                    c1[k] = fmaf(fmaf(chi[k], wiv[sl], cov[k] * wiv[sl]), ivh, 1.0f);
                 } else {
-                    c1 = fmaf(cov[k] * wiv[sl], ivh, 1.0f);
+                    c1[k] = fmaf(cov[k] * wiv[sl], ivh, 1.0f);
                 }
-                if constexpr (EDGE) c1 = (lid0 + k < L - j) ? c1 : -1.0f;
-                const double key = __hiloint2double(__float_as_int(c1), lid0 + k);
+                if constexpr (EDGE) c1[k] = (lid0 + k < L - j) ? c1[k] : -1.0f;
                 cov[k] = fmaf(wdf[sl], dgh, cov[k]);
                 cov[k] = fmaf(dfh, wdg[sl], cov[k]);
-                rk[sl] = key_max(rk[sl], key);
-                ck = (k == 0) ? key : key_max(ck, key);
+                // the row in slot (S + K - 1) % K entered this lane at the end of
+                // the last step: its first cell here starts lm
+                lm[sl] = (k == K - 1) ? c1[k] : c1_max(lm[sl], c1[k]);
             }
-            pck = ck;
-            prk = rk[S];
-            {
-                const uint64_t rb = static_cast<uint64_t>(__double_as_longlong(rk[S]));
-                const int lo = shift_down_i32_zero(static_cast<int>(rb));
-                const int hi = shift_down_i32_zero(static_cast<int>(rb >> 32));
-                rk[S] = __hiloint2double(hi, lo);
-            }
+            pck = c1_max_tree<K>(c1);
+            // the row in slot S leaves this lane: its maximum so far, and this
+            // lane as the argmax's when its cells beat the lanes above (strictly:
+            // the row met those lanes first, at smaller columns)
+            prk = c1_max(rin[S], lm[S]);
+            prl = (lm[S] > rin[S]) ? lane + 1 : tag[S];
+            rin[S] = shift_down_f32(prk, -1.0f);
+            tag[S] = shift_down_i32_zero(prl);
             wdf[S] = shift_down_f32(wdf[S], f.df);
             wdg[S] = shift_down_f32(wdg[S], f.dg);
             wiv[S] = shift_down_f32(wiv[S], f.inv);
@@ -451,14 +487,21 @@ __device__ __forceinline__ void tiles_body(
             flush(c0, EDGE ? min(F, ce - c0) : F);
         }
         // rows still inside the band at the tile's end: slot sl holds row
-        // ce + dbase + lid0 + ((sl - (ce - cb)) mod K)
+        // ce + dbase + lid0 + ((sl - (ce - cb)) mod K); its maximum so far is
+        // max(rin, lm), from this lane when lm is larger.  The slot the last
+        // step refilled holds a row with no cell in this lane yet: its lm still
+        // belongs to the row that left.
+        const int refilled = ((ce - cb - 1) % K + K) % K;
 #pragma unroll
         for (int sl = 0; sl < K; ++sl) {
             const int k = ((sl - (ce - cb)) % K + K) % K;
             const int i = ce + dbase + lid0 + k;
-            if (i < tl && __double2hiint(rk[sl]) > 0) {
-                const int lid = __double2loint(rk[sl]);
-                const unsigned long long gk = global_key(rk[sl], i - dbase - lid);
+            const bool mine = (sl != refilled) && (lm[sl] > rin[sl]);
+            const float v = mine ? lm[sl] : rin[sl];
+            const int rl = mine ? lane : tag[sl] - 1;
+            if (i < tl && rl >= 0 && v > 0.0f) {
+                const int64_t p0 = static_cast<int64_t>(i) - dbase - rl * K;
+                const unsigned long long gk = global_key(v, (p0 << 1) | 1);
                 if (gk > keys[roff + i]) atomicMax(&keys[roff + i], gk);
             }
         }
@@ -491,14 +534,20 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32_wide(MP_TILE_PARAMS) {
 #undef MP_TILE_PARAMS
 #undef MP_TILE_ARGS
 
-// d^2 = 2m(1 - rho): rho from the FP32 key, or recomputed in FP64 from the
-// centred dot product of the winning pair.  One wave per chunk of 64
-// consecutive entries (lane l: entry e0 + l): the series of the chunk's first
-// entry is found once, and a lane whose entry lies in a later series advances
-// from there.  With refine, the dot products run four entries at a time, one
-// 16-lane row per entry (rows reduce with row_ror, so every lane of a row holds
-// the same sum); each lane then finishes its own entry and the wave stores the
-// chunk coalesced.
+// The winning key of an entry names the K candidate partners of one lane's
+// diagonals; their correlations are recomputed in FP64 from centred dot
+// products and the best (the smaller partner on an exact tie) is the index.
+// d^2 = 2m(1 - rho) with that rho (refine) or with the FP32 key's (no refine).
+// One wave per chunk of 64 consecutive entries (lane l: entry e0 + l): the
+// series of the chunk's first entry is found once, and a lane whose entry lies
+// in a later series advances from there.  The dot products run four entries at
+// a time, one 16-lane row per entry: lane r of the row sums positions
+// t = r, r+16, ... of x_t = A[i+t] - mu_i against y_{t+k} = A[lo+t+k] - mu_lo
+// for the K candidates lo .. lo+K-1 (the windows overlap, so the loads of a
+// row stay within a few cache lines), then the row reduces with row_ror, so
+// every lane of a row holds the sums; each lane then finishes its own entry
+// and the wave stores the chunk coalesced.  Candidates past the profile are
+// not loaded (kcount), so no read leaves the series.
 constexpr int kUnpackChunk = 64;
 
 __device__ __forceinline__ double row_sum16(double x) {
@@ -515,6 +564,7 @@ __device__ __forceinline__ double row_sum16(double x) {
     return x;
 }
 
+template <int K>
 __global__ __launch_bounds__(256) void k_unpack_f32r(
     const SeriesDesc* __restrict__ descs, const int64_t* __restrict__ entry_prefix, int nseries,
     int64_t nentries, const double* __restrict__ A, const double* __restrict__ mu,
@@ -539,42 +589,82 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
         const double dm = static_cast<double>(d.m);
         double d2 = 0.0;
         int64_t idx = 0;
-        int p = -1;  // the partner whose distance is recomputed
+        int lo = 0, kcount = 0;  // the candidates are lo .. lo+kcount-1 (and in [lo0, lo0+K))
+        int lo0 = 0;
         if (!live) {
         } else if (low == 0xFFFFFFFFu || !(sigma[ai] > 0.0)) {
             // nothing beat the initial score -1.0, or a flat window (zero
             // variance), whose every score is NaN in FP64 and never wins
             d2 = 2.0 * dm + 2.0;
             idx = index_init;
-        } else if (0xFFFFFFFEu - low >= static_cast<uint32_t>(d.P)) {
-            d2 = __builtin_nan("");  // a partner outside the profile: reported, never dereferenced
-            idx = -2;
         } else {
-            p = static_cast<int>(0xFFFFFFFEu - low);
-            idx = p;
+            const int64_t code = 0xFFFFFFFEu - low;
+            const int64_t p0 = code >> 1;
+            lo0 = static_cast<int>((code & 1) ? p0 - (K - 1) : p0);
+            lo = lo0 < 0 ? 0 : lo0;
+            kcount = static_cast<int>(min<int64_t>(K, static_cast<int64_t>(d.P) - lo));
+            if (kcount < 0) kcount = 0;
             const float c1 = __uint_as_float(static_cast<uint32_t>(key >> 32));
             d2 = 2.0 * dm * (2.0 - static_cast<double>(c1));
+            idx = -2;  // no valid candidate: reported, never dereferenced
         }
-        if (refine && __builtin_amdgcn_ballot_w64(p >= 0) != 0) {
-            const int need = (p >= 0) ? d.m : 0;
-            const int64_t ap = d.off + (p >= 0 ? p : 0);
-            const double mi = (p >= 0) ? mu[ai] : 0.0, mp = (p >= 0) ? mu[ap] : 0.0;
-            double mine = 0.0;
+        if (__builtin_amdgcn_ballot_w64(kcount > 0) != 0) {
+            const double mi = (kcount > 0) ? mu[ai] : 0.0;
+            const double mlo = (kcount > 0) ? mu[d.off + lo] : 0.0;
+            const int64_t alo = d.off + lo;
+            const int need = (kcount > 0) ? d.m : 0;
+            double best_rho = 0.0;
+            int best = -1, first = -1;  // first: a candidate cell, even of a flat window
             for (int u = 0; u < kUnpackChunk / 4; ++u) {
                 const int q = 4 * u + row;  // the entry this row sums in pass u
-                const int qm = __shfl(need, q);
-                const int64_t qi = __shfl(ai, q), qp = __shfl(ap, q);
-                const double qmi = __shfl(mi, q), qmp = __shfl(mp, q);
-                double acc = 0.0;
-                for (int t = rl; t < qm; t += 16) acc = fma(A[qi + t] - qmi, A[qp + t] - qmp, acc);
-                acc = row_sum16(acc);
-                // entry `lane` was summed by row lane % 4 in pass lane / 4
-                const double got = __shfl(acc, 16 * (lane % 4));
-                if (lane / 4 == u) mine = got;
+                const int qm = __shfl(need, q), qk = __shfl(kcount, q);
+                const int64_t qi = __shfl(ai, q), qlo = __shfl(alo, q);
+                const double qmi = __shfl(mi, q), qml = __shfl(mlo, q);
+                double acc[K], sx = 0.0;
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] = 0.0;
+                for (int t = rl; t < qm; t += 16) {
+                    const double x = A[qi + t] - qmi;
+                    sx += x;
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        if (k < qk) acc[k] = fma(x, A[qlo + t + k] - qml, acc[k]);
+                    }
+                }
+                sx = row_sum16(sx);
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] = row_sum16(acc[k]);
+                // entry `lane` was summed by row lane % 4 in pass lane / 4 (the
+                // shuffles run in every lane, the results are used in those)
+                const int src = 16 * (lane % 4);
+                const double ssx = __shfl(sx, src);
+#pragma unroll
+                for (int k = 0; k < K; ++k) acc[k] = __shfl(acc[k], src);
+                if (lane / 4 == u) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const double a = acc[k];
+                        const int p = lo + k;
+                        const int gap = p > i ? p - i : i - p;
+                        if (k < kcount && p >= lo0 && gap >= d.base) {
+                            if (first < 0) first = p;
+                            const double sp = sigma[d.off + p];
+                            const double den = dm * sigma[ai] * sp;
+                            // centred dot: sum x_t (A[p+t] - mu_p) = acc - (mu_p - mu_lo) * sum x_t
+                            const double rho = (a - (mu[d.off + p] - mlo) * ssx) / den;
+                            if (sp > 0.0 && (best < 0 || rho > best_rho)) {
+                                best = p;
+                                best_rho = rho;
+                            }
+                        }
+                    }
+                }
             }
-            if (p >= 0) {
-                const double den = dm * sigma[ai] * sigma[ap];
-                if (den > 0.0) d2 = 2.0 * dm * (1.0 - mine / den);
+            if (best >= 0) {
+                idx = best;
+                if (refine) d2 = 2.0 * dm * (1.0 - best_rho);
+            } else if (first >= 0) {
+                idx = first;  // only flat partners (correlation 0 in FP32): the FP32 value stands
             }
         }
         if (live) {
@@ -692,9 +782,20 @@ int launch_f32r_unpack(const F32rBatch& b, int refine, int64_t index_init, doubl
     const int64_t waves = (b.nentries + kUnpackChunk - 1) / kUnpackChunk;
     const int64_t gw = (waves + 3) / 4;
     const unsigned g = static_cast<unsigned>(gw < 65536 ? gw : 65536);
-    hipLaunchKernelGGL(k_unpack_f32r, dim3(g), dim3(256), 0, st, b.descs, b.entry_prefix,
-                       b.nseries, b.nentries, b.A, b.mu, b.sigma, b.keys, refine, index_init,
-                       profile, index);
+#define MP_UNPACK_CASE(KK)                                                                              \
+    case KK:                                                                                            \
+        hipLaunchKernelGGL(k_unpack_f32r<KK>, dim3(g), dim3(256), 0, st, b.descs, b.entry_prefix,       \
+                           b.nseries, b.nentries, b.A, b.mu, b.sigma, b.keys, refine, index_init,       \
+                           profile, index);                                                             \
+        break;
+    switch (b.band_k) {
+        MP_UNPACK_CASE(4)
+        MP_UNPACK_CASE(8)
+        MP_UNPACK_CASE(16)
+        default:
+            return MP_ERR_BAD_ARG;
+    }
+#undef MP_UNPACK_CASE
     return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
 }
 

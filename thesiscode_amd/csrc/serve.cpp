@@ -475,6 +475,7 @@ int run_tick(mp_server* s, std::unique_lock<std::mutex>& lk) {
         ub.mu = s->mu;
         ub.sigma = s->sigma;
         ub.keys = s->keys;
+        ub.band_k = s->band_k;
         if (rc == MP_OK && ub.nentries > 0) rc = mp::launch_f32r_unpack(ub, s->refine, s->index_init, s->prof, s->idx, fs);
         // one launch copies every finished profile and index to pinned memory
         if (rc == MP_OK)
