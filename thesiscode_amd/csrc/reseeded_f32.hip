@@ -538,31 +538,14 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32_wide(MP_TILE_PARAMS) {
 // diagonals; their correlations are recomputed in FP64 from centred dot
 // products and the best (the smaller partner on an exact tie) is the index.
 // d^2 = 2m(1 - rho) with that rho (refine) or with the FP32 key's (no refine).
-// One wave per chunk of 64 consecutive entries (lane l: entry e0 + l): the
-// series of the chunk's first entry is found once, and a lane whose entry lies
-// in a later series advances from there.  The dot products run four entries at
-// a time, one 16-lane row per entry: lane r of the row sums positions
-// t = r, r+16, ... of x_t = A[i+t] - mu_i against y_{t+k} = A[lo+t+k] - mu_lo
-// for the K candidates lo .. lo+K-1 (the windows overlap, so the loads of a
-// row stay within a few cache lines), then the row reduces with row_ror, so
-// every lane of a row holds the sums; each lane then finishes its own entry
-// and the wave stores the chunk coalesced.  Candidates past the profile are
-// not loaded (kcount), so no read leaves the series.
+// One wave per chunk of 64 consecutive entries, one entry per lane: the series
+// of the chunk's first entry is found once, and a lane whose entry lies in a
+// later series advances from there.  The candidates' windows overlap: a lane
+// walks its window x_t = A[i+t] - mu_i in blocks of K positions, loads the
+// 2K-1 values y = A[lo+t+v] - mu_lo that the K candidates lo .. lo+K-1 need
+// there and does K*K FMAs.  Loads past the last candidate's window (kcount <
+// K near the end of a series) are skipped, so no read leaves the series.
 constexpr int kUnpackChunk = 64;
-
-__device__ __forceinline__ double row_sum16(double x) {
-    auto ror = [](double v, auto ctl_tag) __attribute__((always_inline)) {
-        constexpr int ctl = decltype(ctl_tag)::value;
-        const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), ctl, 0xF, 0xF, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), ctl, 0xF, 0xF, false);
-        return __hiloint2double(hi, lo);
-    };
-    x += ror(x, std::integral_constant<int, 0x128>{});  // row_ror:8
-    x += ror(x, std::integral_constant<int, 0x124>{});  // row_ror:4
-    x += ror(x, std::integral_constant<int, 0x122>{});  // row_ror:2
-    x += ror(x, std::integral_constant<int, 0x121>{});  // row_ror:1
-    return x;
-}
 
 template <int K>
 __global__ __launch_bounds__(256) void k_unpack_f32r(
@@ -573,7 +556,6 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
     const int lane = threadIdx.x % kWave;
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
-    const int row = lane / 16, rl = lane % 16;
     for (int64_t e0 = wave * kUnpackChunk; e0 < nentries; e0 += nwaves * kUnpackChunk) {
         const int64_t e = e0 + lane;
         const bool live = e < nentries;
@@ -589,8 +571,6 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
         const double dm = static_cast<double>(d.m);
         double d2 = 0.0;
         int64_t idx = 0;
-        int lo = 0, kcount = 0;  // the candidates are lo .. lo+kcount-1 (and in [lo0, lo0+K))
-        int lo0 = 0;
         if (!live) {
         } else if (low == 0xFFFFFFFFu || !(sigma[ai] > 0.0)) {
             // nothing beat the initial score -1.0, or a flat window (zero
@@ -600,71 +580,68 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
         } else {
             const int64_t code = 0xFFFFFFFEu - low;
             const int64_t p0 = code >> 1;
-            lo0 = static_cast<int>((code & 1) ? p0 - (K - 1) : p0);
-            lo = lo0 < 0 ? 0 : lo0;
-            kcount = static_cast<int>(min<int64_t>(K, static_cast<int64_t>(d.P) - lo));
-            if (kcount < 0) kcount = 0;
+            const int lo0 = static_cast<int>((code & 1) ? p0 - (K - 1) : p0);
+            const int lo = lo0 < 0 ? 0 : lo0;  // candidates lo .. lo+kcount-1, within [lo0, lo0+K)
+            const int kcount = static_cast<int>(min<int64_t>(K, static_cast<int64_t>(d.P) - lo));
             const float c1 = __uint_as_float(static_cast<uint32_t>(key >> 32));
             d2 = 2.0 * dm * (2.0 - static_cast<double>(c1));
-            idx = -2;  // no valid candidate: reported, never dereferenced
-        }
-        if (__builtin_amdgcn_ballot_w64(kcount > 0) != 0) {
-            const double mi = (kcount > 0) ? mu[ai] : 0.0;
-            const double mlo = (kcount > 0) ? mu[d.off + lo] : 0.0;
-            const int64_t alo = d.off + lo;
-            const int need = (kcount > 0) ? d.m : 0;
-            double best_rho = 0.0;
-            int best = -1, first = -1;  // first: a candidate cell, even of a flat window
-            for (int u = 0; u < kUnpackChunk / 4; ++u) {
-                const int q = 4 * u + row;  // the entry this row sums in pass u
-                const int qm = __shfl(need, q), qk = __shfl(kcount, q);
-                const int64_t qi = __shfl(ai, q), qlo = __shfl(alo, q);
-                const double qmi = __shfl(mi, q), qml = __shfl(mlo, q);
+            idx = -2;  // no candidate cell: reported, never dereferenced
+            if (kcount > 0) {
+                const int m = d.m;
+                const double mi = mu[ai], ml = mu[d.off + lo];
+                const double* __restrict__ x = A + ai;
+                const double* __restrict__ y = A + d.off + lo;
+                const int ylim = m + kcount - 1;  // y positions the candidates read
                 double acc[K], sx = 0.0;
 #pragma unroll
                 for (int k = 0; k < K; ++k) acc[k] = 0.0;
-                for (int t = rl; t < qm; t += 16) {
-                    const double x = A[qi + t] - qmi;
-                    sx += x;
+                int t = 0;
+                for (; t + K <= m; t += K) {
+                    double xs[K], ys[2 * K - 1];
+#pragma unroll
+                    for (int u = 0; u < K; ++u) {
+                        xs[u] = x[t + u] - mi;
+                        sx += xs[u];
+                    }
+#pragma unroll
+                    for (int v = 0; v < 2 * K - 1; ++v) ys[v] = (t + v < ylim) ? y[t + v] - ml : 0.0;
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        if (k < qk) acc[k] = fma(x, A[qlo + t + k] - qml, acc[k]);
+#pragma unroll
+                        for (int u = 0; u < K; ++u) acc[k] = fma(xs[u], ys[u + k], acc[k]);
                     }
                 }
-                sx = row_sum16(sx);
-#pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] = row_sum16(acc[k]);
-                // entry `lane` was summed by row lane % 4 in pass lane / 4 (the
-                // shuffles run in every lane, the results are used in those)
-                const int src = 16 * (lane % 4);
-                const double ssx = __shfl(sx, src);
-#pragma unroll
-                for (int k = 0; k < K; ++k) acc[k] = __shfl(acc[k], src);
-                if (lane / 4 == u) {
+                for (; t < m; ++t) {
+                    const double xv = x[t] - mi;
+                    sx += xv;
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        const double a = acc[k];
-                        const int p = lo + k;
-                        const int gap = p > i ? p - i : i - p;
-                        if (k < kcount && p >= lo0 && gap >= d.base) {
-                            if (first < 0) first = p;
-                            const double sp = sigma[d.off + p];
-                            const double den = dm * sigma[ai] * sp;
-                            // centred dot: sum x_t (A[p+t] - mu_p) = acc - (mu_p - mu_lo) * sum x_t
-                            const double rho = (a - (mu[d.off + p] - mlo) * ssx) / den;
-                            if (sp > 0.0 && (best < 0 || rho > best_rho)) {
-                                best = p;
-                                best_rho = rho;
-                            }
+                        if (k < kcount) acc[k] = fma(xv, y[t + k] - ml, acc[k]);
+                    }
+                }
+                double best_rho = 0.0;
+                int best = -1, first = -1;  // first: a candidate cell, even of a flat window
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int p = lo + k;
+                    const int gap = p > i ? p - i : i - p;
+                    if (k < kcount && p >= lo0 && gap >= d.base) {
+                        if (first < 0) first = p;
+                        const double sp = sigma[d.off + p];
+                        // centred dot: sum x_t (A[p+t] - mu_p) = acc - (mu_p - mu_lo) * sum x_t
+                        const double rho = (acc[k] - (mu[d.off + p] - ml) * sx) / (dm * sigma[ai] * sp);
+                        if (sp > 0.0 && (best < 0 || rho > best_rho)) {
+                            best = p;
+                            best_rho = rho;
                         }
                     }
                 }
-            }
-            if (best >= 0) {
-                idx = best;
-                if (refine) d2 = 2.0 * dm * (1.0 - best_rho);
-            } else if (first >= 0) {
-                idx = first;  // only flat partners (correlation 0 in FP32): the FP32 value stands
+                if (best >= 0) {
+                    idx = best;
+                    if (refine) d2 = 2.0 * dm * (1.0 - best_rho);
+                } else if (first >= 0) {
+                    idx = first;  // only flat partners (correlation 0 in FP32): the FP32 value stands
+                }
             }
         }
         if (live) {
