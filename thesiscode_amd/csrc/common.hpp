@@ -63,6 +63,19 @@ __device__ __forceinline__ int shift_down_i32_zero(int src) {
 __device__ __forceinline__ float shift_down_f32(float src, float fill) {
     return __int_as_float(shift_down_i32(__float_as_int(src), __float_as_int(fill)));
 }
+// Lane 63 receives +0.0.
+__device__ __forceinline__ float shift_down_f32_zero(float src) {
+    return __int_as_float(shift_down_i32_zero(__float_as_int(src)));
+}
+// Lane 63 receives the double whose high word is fill_hi and low word 0 (for
+// -inf: fill_hi = 0xFFF00000).  The low word needs no fill register.
+__device__ __forceinline__ double shift_down_f64_lo0(double src, int fill_hi) {
+    const uint64_t s = static_cast<uint64_t>(__double_as_longlong(src));
+    const int lo = shift_down_i32_zero(static_cast<int>(s));
+    const int hi = shift_down_i32(static_cast<int>(s >> 32), fill_hi);
+    return __longlong_as_double(static_cast<long long>(
+        (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo)));
+}
 
 __device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
 

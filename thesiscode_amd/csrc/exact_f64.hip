@@ -322,7 +322,9 @@ __device__ __forceinline__ double max_tree(const double* sc) {
 // holds them and no wait falls inside a window.  Lanes read slots 8 apart, so
 // the ring is padded by 16 bytes after every 8 slots (slot s at byte
 // 32 s + 16 (s / 8)): the 16 lanes of an LDS cycle then cover all 64 banks.
-template <int K, bool WRITEBACK, bool RING>
+// FEEDLDS (rows by DPP only): the row entering lane 63 comes from LDS staging
+// (below) instead of from a scalar load and eight moves into vector registers.
+template <int K, bool WRITEBACK, bool RING, bool FEEDLDS>
 __device__ __forceinline__ void walk_body(
     const WalkRec* __restrict__ colrec, const WalkRec* __restrict__ rowrec,
     double* qt, unsigned long long* rowval, unsigned long long* rowkey,
@@ -331,10 +333,11 @@ __device__ __forceinline__ void walk_body(
     constexpr int F = kFlush;
     constexpr int BW = kWave * K;
     static_assert(F % K == 0, "flush period must be a multiple of K");
+    static_assert(!(RING && FEEDLDS), "the ring form reads every row from LDS");
     // Column partials, one row of 64 per step, padded by one entry: a flush
     // lane reads 16 consecutive entries of one row (immediate offsets), and
-    // the 16 lanes of an LDS phase then fall on distinct banks.  54,528 bytes
-    // of LDS per group, so that 3 groups still fit in a CU's 160 KiB.
+    // the 16 lanes of an LDS phase then fall on distinct banks.  Without the
+    // ring, 42 KB of LDS per group, so that 3 groups still fit in a CU.
     __shared__ double s_cv[kWavesPerGroup][F][kWave + 1];
     // [0, F): rows leaving lane 0; lane l > 0 writes scratch slot F + l + step
     __shared__ double s_rv[kWavesPerGroup][2 * F + kWave];
@@ -346,6 +349,21 @@ __device__ __forceinline__ void walk_body(
     char* ringb = nullptr;
     WalkRec* stage = nullptr;
     unsigned long long* filt = nullptr;  // [0, F): column results, [F, 2F): row results
+    // FEEDLDS: the rows entering lane 63 (one per step) come from a staging
+    // array in LDS, filled by a direct-to-LDS load one window ahead (two
+    // buffers, alternating by window): a step reads its row with two broadcast
+    // LDS reads into the registers the DPP shift then writes, so lane 63's
+    // fill costs no vector instruction.  The high word of -inf (lane 63's fill
+    // of the running row maximum, in every form) is read from LDS the same way.
+    const WalkRec* feed = nullptr;
+    __shared__ int s_neghi[kWavesPerGroup];
+    const int* const neghi = &s_neghi[threadIdx.x / kWave];
+    if (threadIdx.x % kWave == 0) s_neghi[threadIdx.x / kWave] = static_cast<int>(0xFFF00000u);
+    __builtin_amdgcn_wave_barrier();
+    if constexpr (FEEDLDS) {
+        __shared__ __align__(16) WalkRec s_feed[kWavesPerGroup][2 * F];
+        feed = &s_feed[threadIdx.x / kWave][0];
+    }
     if constexpr (RING) {
         __shared__ __align__(16) char s_ring[kWavesPerGroup][kRingBytes];
         __shared__ WalkRec s_stage[kWavesPerGroup][F];
@@ -506,9 +524,11 @@ __device__ __forceinline__ void walk_body(
             // One column j = c0 + W for all K diagonals of every lane; W is the
             // step's place in the flush window, so the row slot (W mod K) and
             // the LDS offsets are constants.  TAIL: the window may end before
-            // F steps.  Column and feed records are two linear scalar streams.
+            // F steps.  Column records are a linear scalar stream.
             WalkRec cur_c, cur_f;  // the next step's column and feed records
+            int cur_nh = 0;        // the next step's fill word
             int rbase = 0;         // ring byte offset of this window's first refill row
+            const WalkRec* fw = feed;  // this window's staged feed rows (FEEDLDS)
             // A step's column partial and the row leaving lane 0 are written to
             // LDS by the next step, after its wait for the scalar records: the
             // wait (lgkmcnt, which also counts LDS writes) then never waits for
@@ -526,20 +546,23 @@ __device__ __forceinline__ void walk_body(
                 constexpr bool TAIL = decltype(tail_tag)::value;
                 const int j = c0 + W;
                 if (TAIL && j >= L) return false;
-                // Records of this column and of the row entering lane 63 (a
-                // padding record past the triangle; the ring path reads rows
-                // from LDS instead) were loaded one step ago; the next step's
-                // are loaded now, so the scalar loads overlap a step.  Steps
-                // are not interleaved by the scheduler: that keeps each step's
-                // use of its records (and so the wait for them) after the
-                // previous step's loads have had a whole step to land.
+                // The column's record (a scalar load) and the row entering
+                // lane 63 (a padding record past the triangle; a scalar load,
+                // or an LDS read of the staged rows; the ring path reads rows
+                // from LDS instead) were loaded one step ago; the next step's are loaded now, so
+                // the loads overlap a step.  Steps are not interleaved by the
+                // scheduler: that keeps each step's use of its records (and so
+                // the wait for them) after the previous step's loads have had
+                // a whole step to land.
                 __builtin_amdgcn_sched_barrier(0);
                 const WalkRec c = cur_c, f = cur_f;
+                const int fnh = cur_nh;
                 // Scalar loads may return out of order, so a wait for this step's
-                // records waits for every scalar load in flight: take the wait
-                // here (the empty asm needs the values), before the next step's
-                // loads are issued (its memory clobber keeps them below).
-                if constexpr (RING) {
+                // records waits for every scalar and LDS load in flight: take
+                // the wait here (the empty asm needs the values), before the
+                // next step's loads are issued (its memory clobber keeps them
+                // below).
+                if constexpr (RING || FEEDLDS) {
                     asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu) : "memory");
                 } else {
                     asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu), "s"(f.a), "s"(f.am),
@@ -547,7 +570,13 @@ __device__ __forceinline__ void walk_body(
                                  : "memory");
                 }
                 cur_c = colrec[j + 1];
-                if constexpr (!RING) cur_f = rowrec[j + 1 + dbase + BW];
+                if constexpr (!RING && !FEEDLDS) cur_f = rowrec[j + 1 + dbase + BW];
+                // the last step's successor is the next window's first step,
+                // which reads these when the window starts
+                if constexpr (W + 1 < F) {
+                    if constexpr (FEEDLDS) cur_f = fw[W + 1];
+                    cur_nh = *neghi;
+                }
                 // the row this lane takes at the end of the step: row
                 // c0 + W + dbase + (lane+1)*K, which lane+1 finishes now; read
                 // before the step's LDS writes, so the read overlaps the step
@@ -587,7 +616,7 @@ __device__ __forceinline__ void walk_body(
                 // columns); a NaN lm (only masked cells) changes nothing
                 prv = dmax(rin[S], lm[S]);
                 pri = (lm[S] > rin[S]) ? lane + 1 : tag[S];
-                rin[S] = shift_down_f64(prv, NEG);
+                rin[S] = shift_down_f64_lo0(prv, fnh);  // -inf into lane 63
                 tag[S] = shift_down_i32_zero(pri);
                 if constexpr (RING) {
                     wa[S] = nr.a;
@@ -638,6 +667,28 @@ __device__ __forceinline__ void walk_body(
                     }
                     __builtin_amdgcn_wave_barrier();
                     rbase = ring_off((c0 + lid0 + K) % RR);  // a byte offset
+                    cur_nh = *neghi;
+                } else if constexpr (!FEEDLDS) {
+                    cur_nh = *neghi;
+                } else {
+                    // The next window's feed rows (band rows c0 + F + BW + x,
+                    // padding records past the triangle) go to the other
+                    // buffer, 16 bytes per lane; this window's landed before
+                    // the last flush.
+                    __builtin_amdgcn_wave_barrier();
+                    const int nb = (c0 / F) & 1;
+                    if (lane < 2 * F) {
+                        const char* src = reinterpret_cast<const char*>(
+                                              rowrec + (dbase + c0 + F + BW + (lane >> 1))) +
+                                          16 * (lane & 1);
+                        __builtin_amdgcn_global_load_lds(
+                            (const __attribute__((address_space(1))) void*)(src),
+                            (__attribute__((address_space(3))) void*)(feed + (nb ^ 1) * F), 16, 0, 0);
+                    }
+                    fw = feed + nb * F;
+                    cur_f = fw[0];
+                    cur_nh = *neghi;
+                    __builtin_amdgcn_wave_barrier();
                 }
                 bool go = true;
                 [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
@@ -659,19 +710,30 @@ __device__ __forceinline__ void walk_body(
             int c0 = 0;
             uint64_t pc = 0, pr = 0;
             if constexpr (!RING) current(0, pc, pr);
+            if constexpr (FEEDLDS) {
+                // the first window's feed rows (band rows BW + x) to buffer 0
+                __builtin_amdgcn_wave_barrier();
+                if (lane < 2 * F) {
+                    const char* src =
+                        reinterpret_cast<const char*>(rowrec + (dbase + BW + (lane >> 1))) + 16 * (lane & 1);
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
+                                                     (__attribute__((address_space(3))) void*)(feed), 16, 0, 0);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
             cur_c = colrec[0];
-            if constexpr (!RING) cur_f = rowrec[dbase + BW];
+            if constexpr (!RING && !FEEDLDS) cur_f = rowrec[dbase + BW];
             // let the band's set-up loads land here, so the windows never wait on
             // vector memory (whose counter would also cover the flush's atomics)
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
             // whole flush windows: no control flow inside a window
             for (; c0 + F <= L; c0 += F) {
                 window(c0, std::false_type{});
+                // the direct-to-LDS loads of this window (the next rows, and
+                // with the ring the filter values) must have landed: the
+                // compiler does not order LDS reads after them
+                if constexpr (RING || FEEDLDS) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 if constexpr (RING) {
-                    // the direct-to-LDS loads of this window (filter values and
-                    // the next rows) must have landed: the compiler does not
-                    // order LDS reads after them
-                    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                     flush(c0, F, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
                 } else {
                     flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
@@ -682,7 +744,7 @@ __device__ __forceinline__ void walk_body(
             // the last partial window
             if (c0 < L) {
                 window(c0, std::true_type{});
-                if constexpr (RING) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                if constexpr (RING || FEEDLDS) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
             }
         }
@@ -699,25 +761,27 @@ __device__ __forceinline__ void walk_body(
         m, ib, nbands
 
 // K <= 4: at most 168 VGPRs, so that the 3 groups the LDS allows per CU all fit.
+// Its feed rows come from LDS staging (K = 1 at n = 65,536: 9.1 -> 7.4 ms).
 template <int K, bool WRITEBACK>
 __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_f64(
     MP_WALK_PARAMS) {
-    walk_body<K, WRITEBACK, false>(MP_WALK_ARGS);
+    walk_body<K, WRITEBACK, false, true>(MP_WALK_ARGS);
 }
 // K = 8 holds twice the row records (one wave per SIMD); its rows come from the
 // LDS ring, which the one group per CU this leaves has room for.
 template <int K, bool WRITEBACK>
 __global__ __launch_bounds__(kGroup) void k_walk_f64_wide(MP_WALK_PARAMS) {
-    walk_body<K, WRITEBACK, true>(MP_WALK_ARGS);
+    walk_body<K, WRITEBACK, true, false>(MP_WALK_ARGS);
 }
 // K = 8 with its rows shifted down the lanes by DPP instead of the LDS ring:
 // 38 KB of LDS per group and at most 256 VGPRs, so two waves share each SIMD
 // and hide each other's latency; it pays when there are work items for two
-// waves per SIMD.
+// waves per SIMD.  Its feed rows stay scalar loads: staged in LDS (fewer
+// vector instructions) the kernel measured 8% slower at n = 2^22, m = 1024.
 template <int K, bool WRITEBACK>
 __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(2))) void k_walk_f64_pair(
     MP_WALK_PARAMS) {
-    walk_body<K, WRITEBACK, false>(MP_WALK_ARGS);
+    walk_body<K, WRITEBACK, false, false>(MP_WALK_ARGS);
 }
 #undef MP_WALK_PARAMS
 #undef MP_WALK_ARGS

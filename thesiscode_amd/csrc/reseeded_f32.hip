@@ -206,18 +206,23 @@ __device__ __forceinline__ float c1_max_tree(const float* c) {
 // that of a covariance accumulated in FP32, while a step costs FP32
 // instructions only (the fold adds three FP64-class ones per 16 cells).
 //
-// Per step the wave needs two uniform records: the column's terms and the
-// terms of the row entering lane 63.  They are scalar loads issued one step
-// ahead; scalar loads may return out of order, so the wait for them is a wait
-// for every scalar and LDS operation in flight, and the step takes it first,
-// before it issues anything.  The step's column partial and the row leaving
-// lane 0 are written to LDS by the next step, after that wait, so the wait
-// never covers LDS writes issued just before it.
+// Per step the wave needs two uniform records: the column's terms, a scalar
+// load, and the terms of the row entering lane 63, a broadcast LDS read from a
+// staging array that a direct-to-LDS load fills one flush window ahead (two
+// buffers, alternating by window).  The LDS read lands in the registers that
+// the DPP shift of the rows then writes, so lane 63's fill costs no vector
+// instruction.  Both are issued one step ahead; scalar loads may return out of
+// order, so the wait for them is a wait for every scalar and LDS operation in
+// flight, and the step takes it first, before it issues anything.  The step's
+// column partial and the row leaving lane 0 are written to LDS by the next
+// step, after that wait, so the wait never covers LDS writes issued just
+// before it.  fterms is terms again: the staging loads read through it, so that
+// they are known not to write what the scalar loads read.
 template <int K, bool ACC64>
 __device__ __forceinline__ void tiles_body(
     const SeriesDesc* __restrict__ descs, const int64_t* __restrict__ tile_prefix,
     const int64_t* __restrict__ band_prefix, int nseries, const double* __restrict__ A,
-    const double* __restrict__ mu, const TermRec* __restrict__ terms,
+    const double* __restrict__ mu, const TermRec* __restrict__ terms, const TermRec* __restrict__ fterms,
     unsigned long long* keys, unsigned int* counter, int64_t ntiles, int T) {
     using Rec = TermRec;
     constexpr int F = kFlush;
@@ -229,11 +234,13 @@ __device__ __forceinline__ void tiles_body(
     __shared__ double s_buf[kWavesPerGroup][kBufDoubles];
     __shared__ float s_rk[kWavesPerGroup][2 * F + kWave];  // [0, F) row exits; the rest scratch
     __shared__ int s_rl[kWavesPerGroup][2 * F + kWave];    // their lanes + 1 (0: none)
+    __shared__ Rec s_feed[kWavesPerGroup][2 * F];           // rows entering lane 63
 
     const int wv = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     const int lid0 = lane * K;
     double* seeds = &s_buf[wv][0];
+    Rec* const feed = &s_feed[wv][0];
     float* const ck_base = reinterpret_cast<float*>(&s_buf[wv][0]);  // [F][kWave + 1]
 
     for (;;) {
@@ -327,7 +334,8 @@ __device__ __forceinline__ void tiles_body(
         float wiv[K];
         // per row slot: rin = the row's maximum over the lanes above this one,
         // tag = the lane that holds it + 1 (0: none), lm = the maximum over
-        // this lane's cells of the row so far
+        // this lane's cells of the row so far.  rin = 0 means none: a value
+        // c1 <= 0 is never merged, so it needs no lane.
         float rin[K], lm[K];
         int tag[K];
 #pragma unroll
@@ -343,7 +351,7 @@ __device__ __forceinline__ void tiles_body(
             wdf[k] = t.df;
             wdg[k] = t.dg;
             wiv[k] = t.inv;
-            rin[k] = -1.0f;  // below every c1 (>= 0)
+            rin[k] = 0.0f;
             lm[k] = -1.0f;
             tag[k] = 0;
         }
@@ -374,10 +382,25 @@ __device__ __forceinline__ void tiles_body(
             // result is not used)
             const int cs = lane & (F - 1), qq = lane / F;
             const float* part = ck_base + cs * (kWave + 1) + qq * (kWave / 4);
-            C1Lane b{part[0], qq * (kWave / 4)};
+            float pv[kWave / 4];
 #pragma unroll
-            for (int v = 1; v < kWave / 4; ++v) b = c1_better(b, C1Lane{part[v], qq * (kWave / 4) + v});
-            b = c1_better_quarters(b);
+            for (int v = 0; v < kWave / 4; ++v) pv[v] = part[v];
+            // A column can pass the filter only if one of its partials reaches
+            // the c1 of its current key (the key's high word; c1 >= 0 there, so
+            // the order of the words is the order of the values).  The
+            // reduction runs only when some partial of the window can pass.
+            const float thr = __uint_as_float(static_cast<uint32_t>(pc >> 32));
+            bool cand = false;
+#pragma unroll
+            for (int v = 0; v < kWave / 4; ++v) cand |= (pv[v] >= thr);
+            cand = cand && (cs < cnt);
+            C1Lane b{-1.0f, 0};
+            if (__builtin_amdgcn_ballot_w64(cand) != 0) {
+                b = C1Lane{pv[0], qq * (kWave / 4)};
+#pragma unroll
+                for (int v = 1; v < kWave / 4; ++v) b = c1_better(b, C1Lane{pv[v], qq * (kWave / 4) + v});
+                b = c1_better_quarters(b);
+            }
             if (lane < cnt && b.v > 0.0f) {
                 const int jj = jstart + lane;
                 const int64_t p0 = static_cast<int64_t>(d.base) + jj + dbase + b.lane * K;
@@ -396,9 +419,18 @@ __device__ __forceinline__ void tiles_body(
             }
             __builtin_amdgcn_wave_barrier();
         };
-        // The row entering lane 63 at column j, clamped like the others.
-        auto feed_row = [&](int j) __attribute__((always_inline)) { return min(j + dbase + BW, tl - 1); };
+        // The rows entering lane 63 at columns c0 .. c0 + F - 1, clamped like
+        // the others, to feed buffer buf: 16 bytes per lane.
+        auto stage_feed = [&](int c0, int buf) __attribute__((always_inline)) {
+            if (lane < F) {
+                const Rec* src = fterms + roff + min(c0 + lane + dbase + BW, tl - 1);
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
+                                                 (__attribute__((address_space(3))) void*)(feed + buf * F), 16, 0,
+                                                 0);
+            }
+        };
         Rec cur_c, cur_f;  // the next step's column and feed records
+        const Rec* fw = feed;  // this window's feed rows
         float pck = -1.0f, prk = -1.0f;  // the last step's column partial and row exit
         int prl = 0;
         auto put_partials = [&](int w) __attribute__((always_inline)) {
@@ -421,9 +453,10 @@ __device__ __forceinline__ void tiles_body(
             if (TAIL && j >= ce) return false;
             __builtin_amdgcn_sched_barrier(0);
             const Rec c = cur_c, f = cur_f;
-            asm volatile("" ::"s"(c.df), "s"(c.dg), "s"(c.inv), "s"(f.df), "s"(f.dg), "s"(f.inv) : "memory");
+            asm volatile("" ::"s"(c.df), "s"(c.dg), "s"(c.inv) : "memory");
             cur_c = terms[coff + j + 1];
-            cur_f = terms[roff + feed_row(j + 1)];
+            // the window's last step: the next window reads its first row
+            if constexpr (W + 1 < F) cur_f = fw[W + 1];
             if constexpr (W > 0) put_partials(W - 1);
             __builtin_amdgcn_sched_barrier(0);
             const float dfh = c.df, dgh = c.dg;
@@ -434,7 +467,11 @@ __device__ __forceinline__ void tiles_body(
                 const int sl = (S + k) % K;
                 // the same three roundings in both forms: of the covariance to
                 // FP32, of its product with 1/norm_i, and of the final FMA
-                if constexpr (ACC64) {
+                if constexpr (ACC64 && W == 0) {
+                    // the window's first step: cov = 0 (just folded), and
+                    // fma(chi, 1/norm_i, 0) is the product (1/norm_i is finite)
+                    c1[k] = fmaf(chi[k] * wiv[sl], ivh, 1.0f);
+                } else if constexpr (ACC64) {
                     c1[k] = fmaf(fmaf(chi[k], wiv[sl], cov[k] * wiv[sl]), ivh, 1.0f);
                 } else {
                     c1[k] = fmaf(cov[k] * wiv[sl], ivh, 1.0f);
@@ -452,7 +489,7 @@ __device__ __forceinline__ void tiles_body(
             // the row met those lanes first, at smaller columns)
             prk = c1_max(rin[S], lm[S]);
             prl = (lm[S] > rin[S]) ? lane + 1 : tag[S];
-            rin[S] = shift_down_f32(prk, -1.0f);
+            rin[S] = shift_down_f32_zero(prk);
             tag[S] = shift_down_i32_zero(prl);
             wdf[S] = shift_down_f32(wdf[S], f.df);
             wdg[S] = shift_down_f32(wdg[S], f.dg);
@@ -469,6 +506,14 @@ __device__ __forceinline__ void tiles_body(
                 }
             }
             load_filter(c0);
+            // the next window's feed rows to the other buffer; this window's
+            // landed before the last flush
+            const int buf = ((c0 - cb) / F) & 1;
+            __builtin_amdgcn_wave_barrier();
+            stage_feed(c0 + F, buf ^ 1);
+            fw = feed + buf * F;
+            cur_f = fw[0];
+            __builtin_amdgcn_wave_barrier();
             bool go = true;
             [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
                 ((go = go && step(c0, std::integral_constant<int, W>{}, edge_tag, tail_tag)), ...);
@@ -481,8 +526,11 @@ __device__ __forceinline__ void tiles_body(
             }
         };
         cur_c = terms[coff + cb];
-        cur_f = terms[roff + feed_row(cb)];
+        stage_feed(cb, 0);
         for (int c0 = cb; c0 < ce; c0 += F) {
+            // the direct-to-LDS loads in flight (this window's feed rows)
+            // must have landed: the compiler does not order LDS reads after them
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             window(c0, edge_tag, edge_tag);
             flush(c0, EDGE ? min(F, ce - c0) : F);
         }
@@ -517,8 +565,10 @@ __device__ __forceinline__ void tiles_body(
     const SeriesDesc *__restrict__ descs, const int64_t *__restrict__ tile_prefix,              \
         const int64_t *__restrict__ band_prefix, int nseries, const double *__restrict__ A,     \
         const double *__restrict__ mu, const TermRec *__restrict__ terms,                       \
-        unsigned long long *keys, unsigned int *counter, int64_t ntiles, int T
-#define MP_TILE_ARGS descs, tile_prefix, band_prefix, nseries, A, mu, terms, keys, counter, ntiles, T
+        const TermRec *__restrict__ fterms, unsigned long long *keys, unsigned int *counter,    \
+        int64_t ntiles, int T
+#define MP_TILE_ARGS \
+    descs, tile_prefix, band_prefix, nseries, A, mu, terms, fterms, keys, counter, ntiles, T
 
 // K <= 8: at most 128 VGPRs, so that 4 waves share each SIMD.
 template <int K, bool ACC64>
@@ -734,11 +784,13 @@ int launch_f32r_tiles(const F32rBatch& b, int groups, hipStream_t st) {
         if (b.accum64)                                                                          \
             hipLaunchKernelGGL((KERNEL<KK, true>), dim3(groups), dim3(kGroup), 0, st,           \
                                b.descs, b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu,     \
+                               static_cast<const TermRec*>(b.terms),                            \
                                static_cast<const TermRec*>(b.terms), b.keys, b.counter,         \
                                b.ntiles, b.tile_len);                                           \
         else                                                                                    \
             hipLaunchKernelGGL((KERNEL<KK, false>), dim3(groups), dim3(kGroup), 0, st,          \
                                b.descs, b.tile_prefix, b.band_prefix, b.nseries, b.A, b.mu,     \
+                               static_cast<const TermRec*>(b.terms),                            \
                                static_cast<const TermRec*>(b.terms), b.keys, b.counter,         \
                                b.ntiles, b.tile_len);                                           \
         break;
