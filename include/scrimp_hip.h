@@ -56,7 +56,11 @@ typedef struct mp_opts {
                             every 16 columns; within 1e-4 of the FP64 profile. 0 = FP32
                             accumulation, about 10% faster, drifts to ~1e-3 on long
                             near-periodic series                                          */
-    int32_t reserved[2];
+    int32_t walk_form;   /* exact path, band_k = 8: how the band walker runs.  0 = chosen
+                            from the work size; 1 = one wave per SIMD, rows from an LDS
+                            ring; 2 = two waves per SIMD, rows shifted by DPP.  Both give
+                            the same profile values                                     */
+    int32_t reserved;
 } mp_opts;
 
 void mp_opts_default(mp_opts* o);

@@ -55,6 +55,31 @@ def test_band_width_does_not_change_results(band_k):
     assert_indices_match(A, m, p_ref, i_ref, i, abs_tol=_tie_tol(m))
 
 
+_FORM_CASES = {}
+
+
+@pytest.mark.parametrize("walk_form", [1, 2])
+@pytest.mark.parametrize("n,m,seed", [(20000, 64, 31), (65536, 400, 0)])
+def test_band_width_8_walker_forms_bit_exact(walk_form, n, m, seed):
+    # band width 8 in both of its forms, which the automatic choice takes only
+    # at n > 2^20: an LDS ring of rows at one wave per SIMD, DPP rows at two
+    A = tc.random_walk(seed, n)
+    key = (n, m, seed)
+    if key not in _FORM_CASES:
+        _FORM_CASES[key] = _oracle(A, m)
+    p_ref, i_ref = _FORM_CASES[key]
+    print(f"form {walk_form} n={n} m={m}", flush=True)
+    p, i, tm = tc.selfjoin_f64(A, m, band_k=8, walk_form=walk_form)
+    assert tm["band_k"] == 8
+    assert np.array_equal(p, p_ref)
+    assert_indices_match(A, m, p_ref, i_ref, i, abs_tol=_tie_tol(m), min_frac=0.999)
+
+
+def test_walk_form_out_of_range_is_refused():
+    with pytest.raises(tc.ScrimpError):
+        tc.selfjoin_f64(tc.random_walk(1, 2000), 32, band_k=8, walk_form=3)
+
+
 def test_config1_random_walk_bit_exact():
     # n = 65,536 random walk (seed 0), m = 400, exclusion m/4: the CPU-runnable configuration
     A = tc.random_walk(0, 65536)

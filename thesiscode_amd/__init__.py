@@ -47,7 +47,7 @@ def cells(n: int, m: int, excl: int | None = None) -> int:
 
 
 def make_opts(exclusion: int = -1, band_k: int = 0, tile_len: int = 0, refine: bool = True,
-              index_init: int = -1, accum64: bool = True) -> Opts:
+              index_init: int = -1, accum64: bool = True, walk_form: int = 0) -> Opts:
     o = Opts()
     lib().mp_opts_default(C.byref(o))
     o.exclusion = exclusion
@@ -56,6 +56,7 @@ def make_opts(exclusion: int = -1, band_k: int = 0, tile_len: int = 0, refine: b
     o.refine = 1 if refine else 0
     o.index_init = index_init
     o.accum64 = 1 if accum64 else 0
+    o.walk_form = walk_form
     return o
 
 
@@ -71,9 +72,10 @@ def _selfjoin(fn_name: str, ts, m: int, opts: Opts):
     return prof, idx, tm.as_dict()
 
 
-def selfjoin_f64(ts, m: int, *, exclusion: int = -1, band_k: int = 0, index_init: int = -1):
-    """Plain FP64 self-join, bit-identical to the reference arithmetic."""
-    return _selfjoin("mp_selfjoin_f64", ts, m, make_opts(exclusion, band_k, 0, True, index_init))
+def selfjoin_f64(ts, m: int, *, exclusion: int = -1, band_k: int = 0, index_init: int = -1, walk_form: int = 0):
+    """Plain FP64 self-join, bit-identical to the reference arithmetic (walk_form: see mp_opts)."""
+    return _selfjoin("mp_selfjoin_f64", ts, m,
+                     make_opts(exclusion, band_k, 0, True, index_init, walk_form=walk_form))
 
 
 def selfjoin_f32r(ts, m: int, *, exclusion: int = -1, band_k: int = 0, tile_len: int = 0,

@@ -26,7 +26,8 @@ class ScrimpError(RuntimeError):
 class Opts(C.Structure):
     _fields_ = [
         ("exclusion", C.c_int32), ("band_k", C.c_int32), ("tile_len", C.c_int32),
-        ("refine", C.c_int32), ("index_init", C.c_int32), ("accum64", C.c_int32), ("reserved", C.c_int32 * 2),
+        ("refine", C.c_int32), ("index_init", C.c_int32), ("accum64", C.c_int32), ("walk_form", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

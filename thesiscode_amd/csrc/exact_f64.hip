@@ -936,7 +936,7 @@ int launch_walk_f64(const WalkF64Launch& a, hipStream_t st) {
                                a.qt, a.rowval, a.rowkey, a.colval, a.colkey, a.counter,          \
                                a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib, nbands);       \
         break;
-    if (a.band_k == 8 && a.pair) {
+    if (a.band_k == 8 && a.form == 2) {
         switch (a.band_k) { MP_WALK_CASE(8, k_walk_f64_pair) }
     } else {
         switch (a.band_k) {

@@ -72,7 +72,9 @@ struct WalkF64Launch {
     unsigned int* counter;
     int64_t row_off, col_off;
     int blocklen, tl, m, ib, writeback, band_k, groups;
-    int pair;               // band_k = 8: two waves per SIMD, rows by DPP (k_walk_f64_pair)
+    int form;               // band_k = 8: 1 rows from an LDS ring, one wave per SIMD
+                            // (k_walk_f64_wide); 2 rows by DPP, two waves per SIMD
+                            // (k_walk_f64_pair)
 };
 int walk_f64_nbands(int blocklen, int band_k);
 int launch_walk_f64(const WalkF64Launch& a, hipStream_t st);
