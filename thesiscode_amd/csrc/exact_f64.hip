@@ -261,19 +261,13 @@ __global__ void k_pack_rec(const double* __restrict__ A, const double* __restric
     out[i] = r;
 }
 
-// (value, lane) merge: the larger score wins; exact ties keep the smaller lane
-// (the smaller diagonal, so the smaller partner, as the result keys rank exact
-// ties).  NaN (a masked cell) ranks below every number, so a NaN operand never
-// absorbs a score and the merge is a maximum over a total order: any reduction
-// order gives the same result.
+// A column result: its exact maximum score and the lane that holds it (the
+// smallest such lane: the smaller diagonal, so the smaller partner, as the
+// result keys rank exact ties); id -1: every partial was a masked cell (NaN).
 struct ScoreId {
     double v;
     int id;
 };
-__device__ __forceinline__ ScoreId better(ScoreId a, ScoreId b) {
-    const bool take = b.v > a.v || (a.v != a.v) || (b.v == a.v && b.id < a.id);
-    return take ? b : a;
-}
 // Maximum of two scores with the plain instruction: it returns the other
 // operand when one is a quiet NaN (a masked cell), and the scores here carry
 // no signalling NaNs, so the per-operand quieting a generic fmax adds is not
@@ -284,19 +278,27 @@ __device__ __forceinline__ double dmax(double a, double b) {
     return r;
 }
 
-// better() over lanes l, l^16, l^32 and l^48, in every lane.  Each gfx950 row
-// swap leaves a lane with its own dword and the dword of lane l^32 (l^16) in
-// two registers, in some order; better() is symmetric, so the order is free.
-__device__ __forceinline__ ScoreId better_quarters(ScoreId x) {
-    auto pair = [](ScoreId v, auto swap) __attribute__((always_inline)) {
-        const auto lo = swap(static_cast<unsigned>(__double2loint(v.v)));
-        const auto hi = swap(static_cast<unsigned>(__double2hiint(v.v)));
-        const auto id = swap(static_cast<unsigned>(v.id));
-        return better(ScoreId{__hiloint2double(static_cast<int>(hi[0]), static_cast<int>(lo[0])), static_cast<int>(id[0])},
-                      ScoreId{__hiloint2double(static_cast<int>(hi[1]), static_cast<int>(lo[1])), static_cast<int>(id[1])});
+// dmax() and min() over lanes l, l^16, l^32 and l^48, in every lane.  Each
+// gfx950 row swap leaves a lane with its own dword and the dword of lane l^32
+// (l^16) in two registers, in some order; both operations are symmetric, so
+// the order is free.
+__device__ __forceinline__ double max_quarters(double x) {
+    auto step = [](double v, auto swap) __attribute__((always_inline)) {
+        const auto lo = swap(static_cast<unsigned>(__double2loint(v)));
+        const auto hi = swap(static_cast<unsigned>(__double2hiint(v)));
+        return dmax(__hiloint2double(static_cast<int>(hi[0]), static_cast<int>(lo[0])),
+                    __hiloint2double(static_cast<int>(hi[1]), static_cast<int>(lo[1])));
     };
-    x = pair(x, [](unsigned v) { return __builtin_amdgcn_permlane32_swap(v, v, false, false); });
-    return pair(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
+    x = step(x, [](unsigned v) { return __builtin_amdgcn_permlane32_swap(v, v, false, false); });
+    return step(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
+}
+__device__ __forceinline__ int min_quarters(int x) {
+    auto step = [](int v, auto swap) __attribute__((always_inline)) {
+        const auto s = swap(static_cast<unsigned>(v));
+        return min(static_cast<int>(s[0]), static_cast<int>(s[1]));
+    };
+    x = step(x, [](unsigned v) { return __builtin_amdgcn_permlane32_swap(v, v, false, false); });
+    return step(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
 }
 
 // Maximum of N consecutive cells starting at cell K0 (pairwise tree).
@@ -406,10 +408,8 @@ __device__ __forceinline__ void walk_body(
         // lanes 16 qq .. 16 qq + 15).  A column result passes the filter iff
         // its score reaches thr, the score of the truncated current value (the
         // order map is monotone), so a column can pass only if one of its
-        // partials does.  The reduction (a pairwise tree, then the other
-        // quarters; better() is a maximum over (score, lane), so the order is
-        // free) runs only when some partial of the window passes.  For
-        // cs >= cnt the quarter is stale and unused.
+        // partials does.  The reduction runs only when some partial of the
+        // window passes.  For cs >= cnt the quarter is stale and unused.
         const int cs = lane & (F - 1), qq = lane / F;
         const double* const pv = &s_cv[wv][cs][qq * (kWave / 4)];
         double v[kWave / 4];
@@ -429,16 +429,18 @@ __device__ __forceinline__ void walk_body(
         uint64_t oc = 0;
         ScoreId b{NEG, -1};
         if (__builtin_amdgcn_ballot_w64(cand) != 0) {
-            ScoreId c[kWave / 4];
+            // the column's exact maximum (a NaN partial is a masked cell, and
+            // v_max_f64 returns the other operand), then the smallest lane
+            // holding it: this quarter's first, then the smallest over the
+            // quarters
+            double mx = max_tree<kWave / 4>(v);
+            mx = max_quarters(mx);
+            int id = kWave;
 #pragma unroll
-            for (int u = 0; u < kWave / 4; ++u) c[u] = ScoreId{v[u], qq * (kWave / 4) + u};
-#pragma unroll
-            for (int w = kWave / 8; w >= 1; w >>= 1) {
-#pragma unroll
-                for (int u = 0; u < w; ++u) c[u] = better(c[u], c[u + w]);
-            }
-            b = better_quarters(c[0]);
-            oc = ord_f64(b.v);
+            for (int u = kWave / 4 - 1; u >= 0; --u) id = (v[u] == mx) ? qq * (kWave / 4) + u : id;
+            id = min_quarters(id);
+            b = ScoreId{mx, id < kWave ? id : -1};
+            oc = ord_f64(mx);
         }
         // (non-short-circuit forms: both tests are done before the reload)
         // a NaN result: every partial of the column was a masked cell
