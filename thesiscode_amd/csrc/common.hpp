@@ -63,6 +63,14 @@ __device__ __forceinline__ int shift_down_i32_zero(int src) {
 __device__ __forceinline__ float shift_down_f32(float src, float fill) {
     return __int_as_float(shift_down_i32(__float_as_int(src), __float_as_int(fill)));
 }
+// A wave-uniform double (scalar registers) into vector registers with one
+// 64-bit move, where the compiler would issue two 32-bit moves: the fill value
+// of a shift_down_f64, whose DPP moves then overwrite lanes 0..62 in place.
+__device__ __forceinline__ double vmov64(double s) {
+    double d;
+    asm("v_mov_b64 %0, %1" : "=v"(d) : "s"(s));
+    return d;
+}
 // Lane 63 receives +0.0.
 __device__ __forceinline__ float shift_down_f32_zero(float src) {
     return __int_as_float(shift_down_i32_zero(__float_as_int(src)));

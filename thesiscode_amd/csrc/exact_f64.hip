@@ -626,10 +626,19 @@ __device__ __forceinline__ void walk_body(
                     ws[S] = nr.s;
                     wm[S] = nr.mu;
                 } else {
-                    wa[S] = shift_down_f64(wa[S], f.a);
-                    wam[S] = shift_down_f64(wam[S], f.am);
-                    ws[S] = shift_down_f64(ws[S], f.s);
-                    wm[S] = shift_down_f64(wm[S], f.mu);
+                    // FEEDLDS: f is in vector registers already; else one
+                    // 64-bit move per double brings it there
+                    if constexpr (FEEDLDS) {
+                        wa[S] = shift_down_f64(wa[S], f.a);
+                        wam[S] = shift_down_f64(wam[S], f.am);
+                        ws[S] = shift_down_f64(ws[S], f.s);
+                        wm[S] = shift_down_f64(wm[S], f.mu);
+                    } else {
+                        wa[S] = shift_down_f64(wa[S], vmov64(f.a));
+                        wam[S] = shift_down_f64(wam[S], vmov64(f.am));
+                        ws[S] = shift_down_f64(ws[S], vmov64(f.s));
+                        wm[S] = shift_down_f64(wm[S], vmov64(f.mu));
+                    }
                 }
                 return true;
             };
