@@ -421,19 +421,16 @@ __device__ __forceinline__ void walk_body(
             pr = filt[F + (lane & (F - 1))];
         }
         const double thr = unord_f64((pc >> ib) << ib);
-        bool cand = false;
-#pragma unroll
-        for (int u = 0; u < kWave / 4; ++u) cand |= (v[u] >= thr);
-        cand = cand && (cs < cnt);
+        // the quarter's exact maximum (a NaN partial is a masked cell, and
+        // v_max_f64 returns the other operand; all NaN: NaN, which fails)
+        double mx = max_tree<kWave / 4>(v);
+        const bool cand = (mx >= thr) && (cs < cnt);
         const int jj = jstart + lane;
         uint64_t oc = 0;
         ScoreId b{NEG, -1};
         if (__builtin_amdgcn_ballot_w64(cand) != 0) {
-            // the column's exact maximum (a NaN partial is a masked cell, and
-            // v_max_f64 returns the other operand), then the smallest lane
-            // holding it: this quarter's first, then the smallest over the
-            // quarters
-            double mx = max_tree<kWave / 4>(v);
+            // the column's maximum, then the smallest lane holding it: this
+            // quarter's first, then the smallest over the quarters
             mx = max_quarters(mx);
             int id = kWave;
 #pragma unroll

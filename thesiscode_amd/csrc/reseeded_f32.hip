@@ -189,12 +189,25 @@ __device__ __forceinline__ C1Lane c1_better_quarters(C1Lane x) {
     return pair(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
 }
 
+// Maximum of three c1 values in one instruction.
+__device__ __forceinline__ float c1_max3(float a, float b, float c) {
+    float r;
+    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+// Maximum of N consecutive c1 values starting at K0: three-input maxima where
+// they fit (8 values: 3 v_max3 + 1 v_max, where pairwise maxima take 7).
 template <int N, int K0 = 0>
 __device__ __forceinline__ float c1_max_tree(const float* c) {
     if constexpr (N == 1) {
         return c[K0];
+    } else if constexpr (N == 2) {
+        return c1_max(c[K0], c[K0 + 1]);
+    } else if constexpr (N == 3) {
+        return c1_max3(c[K0], c[K0 + 1], c[K0 + 2]);
     } else {
-        return c1_max(c1_max_tree<N / 2, K0>(c), c1_max_tree<N - N / 2, K0 + N / 2>(c));
+        constexpr int A = (N + 2) / 3, B = (N - A + 1) / 2;
+        return c1_max3(c1_max_tree<A, K0>(c), c1_max_tree<B, K0 + A>(c), c1_max_tree<N - A - B, K0 + A + B>(c));
     }
 }
 
@@ -390,10 +403,7 @@ __device__ __forceinline__ void tiles_body(
             // the order of the words is the order of the values).  The
             // reduction runs only when some partial of the window can pass.
             const float thr = __uint_as_float(static_cast<uint32_t>(pc >> 32));
-            bool cand = false;
-#pragma unroll
-            for (int v = 0; v < kWave / 4; ++v) cand |= (pv[v] >= thr);
-            cand = cand && (cs < cnt);
+            const bool cand = (c1_max_tree<kWave / 4>(pv) >= thr) && (cs < cnt);
             C1Lane b{-1.0f, 0};
             if (__builtin_amdgcn_ballot_w64(cand) != 0) {
                 b = C1Lane{pv[0], qq * (kWave / 4)};
