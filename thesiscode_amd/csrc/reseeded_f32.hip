@@ -166,29 +166,30 @@ __device__ __forceinline__ float c1_max(float a, float b) {
     return r;
 }
 
-// (c1, lane): the larger c1 wins, the smaller lane on exact ties.
+// A column result: its maximum c1 and the smallest lane that holds it.
 struct C1Lane {
     float v;
     int lane;
 };
-__device__ __forceinline__ C1Lane c1_better(C1Lane a, C1Lane b) {
-    const bool take = (b.v > a.v) | ((b.v == a.v) & (b.lane < a.lane));
-    return take ? b : a;
-}
-// c1_better() over lanes l, l^16, l^32 and l^48, in every lane.  Each gfx950
-// row swap leaves a lane with its own dword and the dword of lane l^32 (l^16)
-// in two registers, in some order; the result does not depend on it.
-__device__ __forceinline__ C1Lane c1_better_quarters(C1Lane x) {
-    auto pair = [](C1Lane v, auto swap) __attribute__((always_inline)) {
-        const auto a = swap(__float_as_uint(v.v));
-        const auto l = swap(static_cast<unsigned>(v.lane));
-        return c1_better(C1Lane{__uint_as_float(a[0]), static_cast<int>(l[0])},
-                         C1Lane{__uint_as_float(a[1]), static_cast<int>(l[1])});
+// c1_max() and min() over lanes l, l^16, l^32 and l^48, in every lane.  Each
+// gfx950 row swap leaves a lane with its own dword and the dword of lane l^32
+// (l^16) in two registers, in some order; both operations are symmetric.
+__device__ __forceinline__ float c1_max_quarters(float x) {
+    auto step = [](float v, auto swap) __attribute__((always_inline)) {
+        const auto a = swap(__float_as_uint(v));
+        return c1_max(__uint_as_float(a[0]), __uint_as_float(a[1]));
     };
-    x = pair(x, [](unsigned v) { return __builtin_amdgcn_permlane32_swap(v, v, false, false); });
-    return pair(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
+    x = step(x, [](unsigned v) { return __builtin_amdgcn_permlane32_swap(v, v, false, false); });
+    return step(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
 }
-
+__device__ __forceinline__ int c1_min_quarters(int x) {
+    auto step = [](int v, auto swap) __attribute__((always_inline)) {
+        const auto a = swap(static_cast<unsigned>(v));
+        return min(static_cast<int>(a[0]), static_cast<int>(a[1]));
+    };
+    x = step(x, [](unsigned v) { return __builtin_amdgcn_permlane32_swap(v, v, false, false); });
+    return step(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
+}
 // Maximum of three c1 values in one instruction.
 __device__ __forceinline__ float c1_max3(float a, float b, float c) {
     float r;
@@ -403,13 +404,17 @@ __device__ __forceinline__ void tiles_body(
             // the order of the words is the order of the values).  The
             // reduction runs only when some partial of the window can pass.
             const float thr = __uint_as_float(static_cast<uint32_t>(pc >> 32));
-            const bool cand = (c1_max_tree<kWave / 4>(pv) >= thr) && (cs < cnt);
+            const float qmx = c1_max_tree<kWave / 4>(pv);  // this quarter's maximum
+            const bool cand = (qmx >= thr) && (cs < cnt);
             C1Lane b{-1.0f, 0};
             if (__builtin_amdgcn_ballot_w64(cand) != 0) {
-                b = C1Lane{pv[0], qq * (kWave / 4)};
+                // the column's maximum, then the smallest lane holding it: this
+                // quarter's first, then the smallest over the quarters
+                const float mx = c1_max_quarters(qmx);
+                int id = kWave;
 #pragma unroll
-                for (int v = 1; v < kWave / 4; ++v) b = c1_better(b, C1Lane{pv[v], qq * (kWave / 4) + v});
-                b = c1_better_quarters(b);
+                for (int v = kWave / 4 - 1; v >= 0; --v) id = (pv[v] == mx) ? qq * (kWave / 4) + v : id;
+                b = C1Lane{mx, c1_min_quarters(id)};
             }
             if (lane < cnt && b.v > 0.0f) {
                 const int jj = jstart + lane;
