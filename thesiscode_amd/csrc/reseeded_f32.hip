@@ -495,15 +495,21 @@ __device__ __forceinline__ void tiles_body(
                 cov[k] = fmaf(wdf[sl], dgh, cov[k]);
                 cov[k] = fmaf(dfh, wdg[sl], cov[k]);
                 // the row in slot (S + K - 1) % K entered this lane at the end of
-                // the last step: its first cell here starts lm
-                lm[sl] = (k == K - 1) ? c1[k] : c1_max(lm[sl], c1[k]);
+                // the last step: its first cell here starts lm; the row in slot
+                // S (k = 0) leaves now, its last cell taken below
+                if (k == K - 1) {
+                    lm[sl] = c1[k];
+                } else if (k > 0) {
+                    lm[sl] = c1_max(lm[sl], c1[k]);
+                }
             }
             pck = c1_max_tree<K>(c1);
-            // the row in slot S leaves this lane: its maximum so far, and this
-            // lane as the argmax's when its cells beat the lanes above (strictly:
-            // the row met those lanes first, at smaller columns)
-            prk = c1_max(rin[S], lm[S]);
-            prl = (lm[S] > rin[S]) ? lane + 1 : tag[S];
+            // the row in slot S leaves this lane: its maximum so far (with its
+            // last cell here, c1[0]), and this lane as the argmax's when its
+            // cells beat the lanes above (strictly: the row met those lanes
+            // first, at smaller columns)
+            prk = c1_max3(rin[S], lm[S], c1[0]);
+            prl = (prk > rin[S]) ? lane + 1 : tag[S];
             rin[S] = shift_down_f32_zero(prk);
             tag[S] = shift_down_i32_zero(prl);
             wdf[S] = shift_down_f32(wdf[S], f.df);
