@@ -459,9 +459,19 @@ __device__ __forceinline__ void tiles_body(
         // EDGE: the window reaches past the triangle; a cell (j, lid) lies in it
         // iff j + lid < L, and a cell outside gets the key -1 (never a winner:
         // valid keys are corr + 1 >= 0).  TAIL: the window may end before F steps.
+        // Row maxima two steps at a time: an even step (the first of a pair;
+        // K and F are even, so a pair never spans windows) leaves its cells of
+        // rows that stay in the lane in pend[k], and the odd step folds each
+        // with its own cell of the row in one v_max3 (9 maxima per pair where
+        // one per cell took 14).
+        float pend[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) pend[k] = -1.0f;
         auto step = [&](int c0, auto w_tag, auto edge_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
             constexpr int W = decltype(w_tag)::value;
             constexpr int S = W % K;
+            constexpr bool ODD = (W & 1) != 0;
+            static_assert(K % 2 == 0 && F % 2 == 0, "steps pair up within a window");
             constexpr bool EDGE = decltype(edge_tag)::value;
             constexpr bool TAIL = decltype(tail_tag)::value;
             const int j = c0 + W;
@@ -496,19 +506,33 @@ __device__ __forceinline__ void tiles_body(
                 cov[k] = fmaf(dfh, wdg[sl], cov[k]);
                 // the row in slot (S + K - 1) % K entered this lane at the end of
                 // the last step: its first cell here starts lm; the row in slot
-                // S (k = 0) leaves now, its last cell taken below
+                // S (k = 0) leaves now, its last cell taken below.  The others
+                // wait in pend (even step) or fold the pending cell of the
+                // last step, which was their cell k + 1 (odd step; k = K - 2:
+                // that one was the row's first, already in lm).
                 if (k == K - 1) {
                     lm[sl] = c1[k];
                 } else if (k > 0) {
-                    lm[sl] = c1_max(lm[sl], c1[k]);
+                    if constexpr (!ODD) {
+                        pend[k] = c1[k];
+                    } else if (k == K - 2) {
+                        lm[sl] = c1_max(lm[sl], c1[k]);
+                    } else {
+                        lm[sl] = c1_max3(lm[sl], pend[k + 1], c1[k]);
+                    }
                 }
             }
             pck = c1_max_tree<K>(c1);
             // the row in slot S leaves this lane: its maximum so far (with its
-            // last cell here, c1[0]), and this lane as the argmax's when its
+            // cells not yet in lm: this step's c1[0] and, after an even step,
+            // the last step's pend[1]), and this lane as the argmax's when its
             // cells beat the lanes above (strictly: the row met those lanes
             // first, at smaller columns)
-            prk = c1_max3(rin[S], lm[S], c1[0]);
+            if constexpr (!ODD) {
+                prk = c1_max3(rin[S], lm[S], c1[0]);
+            } else {
+                prk = c1_max3(rin[S], c1_max(lm[S], pend[1]), c1[0]);
+            }
             prl = (prk > rin[S]) ? lane + 1 : tag[S];
             rin[S] = shift_down_f32_zero(prk);
             tag[S] = shift_down_i32_zero(prl);
@@ -541,7 +565,21 @@ __device__ __forceinline__ void tiles_body(
             }(std::make_integer_sequence<int, F>{});
             // the window's last step has no successor to write its partials
             if constexpr (decltype(tail_tag)::value) {
-                put_partials(min(F, ce - c0) - 1);
+                const int cnt = min(F, ce - c0);
+                put_partials(cnt - 1);
+                // a window cut after an even step: that step's pending cells
+                // go into lm now (its slot is a compile-time constant per case)
+                if (cnt & 1) {
+                    const int s_last = (cnt - 1) % K;
+                    [&]<int... I>(std::integer_sequence<int, I...>) __attribute__((always_inline)) {
+                        auto fold = [&](auto s_tag) __attribute__((always_inline)) {
+                            constexpr int SL = decltype(s_tag)::value;
+#pragma unroll
+                            for (int k = 1; k < K - 1; ++k) lm[(SL + k) % K] = c1_max(lm[(SL + k) % K], pend[k]);
+                        };
+                        ((s_last == 2 * I ? fold(std::integral_constant<int, 2 * I>{}) : void()), ...);
+                    }(std::make_integer_sequence<int, K / 2>{});
+                }
             } else {
                 put_partials(F - 1);
             }
