@@ -381,10 +381,18 @@ __device__ __forceinline__ void tiles_body(
         // row).  Keys only grow, so a stale read only lets an unneeded atomic
         // through.
         unsigned long long pc = 0, pr = 0;
-        auto load_filter = [&](int c0) __attribute__((always_inline)) {
-            const int jj = min(c0 + (lane & (F - 1)), ce - 1);
-            pc = keys[coff + jj];
-            pr = keys[roff + jj + dbase];
+        // (a main tile's windows are whole: no clamp, and the window's base is
+        // uniform, so the loads take a scalar base and a fixed lane offset)
+        const int lane_f = lane & (F - 1);
+        auto load_filter = [&](int c0, auto edge_tag) __attribute__((always_inline)) {
+            if constexpr (decltype(edge_tag)::value) {
+                const int jj = min(c0 + lane_f, ce - 1);
+                pc = keys[coff + jj];
+                pr = keys[roff + jj + dbase];
+            } else {
+                pc = (keys + (coff + c0))[lane_f];
+                pr = (keys + (roff + c0 + dbase))[lane_f];
+            }
         };
         // Column results hold p0 = the partner of the lane's first diagonal
         // (candidates p0 + k); row results p0 = the column of that diagonal
@@ -416,7 +424,7 @@ __device__ __forceinline__ void tiles_body(
                 for (int v = kWave / 4 - 1; v >= 0; --v) id = (pv[v] == mx) ? qq * (kWave / 4) + v : id;
                 b = C1Lane{mx, c1_min_quarters(id)};
             }
-            if (lane < cnt && b.v > 0.0f) {
+            if (lane < cnt && b.v > 0.0f && b.v >= thr) {
                 const int jj = jstart + lane;
                 const int64_t p0 = static_cast<int64_t>(d.base) + jj + dbase + b.lane * K;
                 const unsigned long long gk = global_key(b.v, p0 << 1);
@@ -425,7 +433,9 @@ __device__ __forceinline__ void tiles_body(
             if (lane < cnt) {
                 const float rkv = s_rk[wv][lane];
                 const int rl = s_rl[wv][lane] - 1;
-                if (rl >= 0 && rkv > 0.0f) {
+                // (the c1 of the row's current key first: most rows cannot
+                // pass, and then no lane builds a key)
+                if (rl >= 0 && rkv > 0.0f && rkv >= __uint_as_float(static_cast<uint32_t>(pr >> 32))) {
                     const int jj = jstart + lane;
                     const int64_t p0 = static_cast<int64_t>(jj) - rl * K;
                     const unsigned long long gk = global_key(rkv, (p0 << 1) | 1);
@@ -550,7 +560,7 @@ __device__ __forceinline__ void tiles_body(
                     chi[k] = static_cast<float>(covd[k]);
                 }
             }
-            load_filter(c0);
+            load_filter(c0, edge_tag);
             // the next window's feed rows to the other buffer; this window's
             // landed before the last flush
             const int buf = ((c0 - cb) / F) & 1;
