@@ -58,8 +58,10 @@ typedef struct mp_opts {
                             near-periodic series                                          */
     int32_t walk_form;   /* exact path, band_k = 8: how the band walker runs.  0 = chosen
                             from the work size; 1 = one wave per SIMD, rows from an LDS
-                            ring; 2 = two waves per SIMD, rows shifted by DPP.  Both give
-                            the same profile values                                     */
+                            ring; 2 = two waves per SIMD, rows shifted by DPP; 3 = two
+                            waves per SIMD, rows in place and the diagonals' running
+                            dot products rotated by DPP.  All give the same profile
+                            values                                                      */
     int32_t reserved;
 } mp_opts;
 

@@ -58,11 +58,12 @@ def test_band_width_does_not_change_results(band_k):
 _FORM_CASES = {}
 
 
-@pytest.mark.parametrize("walk_form", [1, 2])
+@pytest.mark.parametrize("walk_form", [1, 2, 3])
 @pytest.mark.parametrize("n,m,seed", [(20000, 64, 31), (65536, 400, 0)])
 def test_band_width_8_walker_forms_bit_exact(walk_form, n, m, seed):
-    # band width 8 in both of its forms, which the automatic choice takes only
-    # at n > 2^20: an LDS ring of rows at one wave per SIMD, DPP rows at two
+    # band width 8 in each of its forms, which the automatic choice takes only
+    # at n > 2^20: an LDS ring of rows at one wave per SIMD, DPP rows at two,
+    # rows in place with the diagonals rotating at two
     A = tc.random_walk(seed, n)
     key = (n, m, seed)
     if key not in _FORM_CASES:
@@ -77,7 +78,7 @@ def test_band_width_8_walker_forms_bit_exact(walk_form, n, m, seed):
 
 def test_walk_form_out_of_range_is_refused():
     with pytest.raises(tc.ScrimpError):
-        tc.selfjoin_f64(tc.random_walk(1, 2000), 32, band_k=8, walk_form=3)
+        tc.selfjoin_f64(tc.random_walk(1, 2000), 32, band_k=8, walk_form=4)
 
 
 def test_config1_random_walk_bit_exact():

@@ -114,11 +114,12 @@ int geometry(int64_t n, int32_t m, const mp_opts* o, Geometry* g) {
 // 2.19, 1.30, 0.75 (K = 1, 2, 4) and 0.38 for K = 8 with its LDS row ring;
 // s waves run at s * g(s) times the single-wave cost per wave, g = 1, 0.69,
 // 0.57, 0.49 for s = 1..4; K = 8 at two waves per SIMD (used when there are
-// more items than SIMDs): 0.637 per wave.  Pick the K that minimises it.
-// `form` (mp_opts.walk_form) picks the K = 8 form; 0 chooses it.
+// more items than SIMDs, in its rotating form): 0.608 per wave.  Pick the K
+// that minimises it.  `form` (mp_opts.walk_form) picks the K = 8 form; 0
+// chooses the LDS ring at one wave per SIMD, the rotating form at two.
 struct WalkChoice {
     int band_k;
-    int form;  // K = 8: the launch's form (1 LDS ring, 2 DPP rows at two waves per SIMD)
+    int form;  // K = 8: the launch's form (1 LDS ring, 2 DPP rows at two waves per SIMD, 3 rotating diagonals)
 };
 WalkChoice pick_band_k_f64(const mp_ctx* c, int W, int requested, int form) {
     const int64_t S = 4LL * std::max(1, c->cus);
@@ -126,7 +127,7 @@ WalkChoice pick_band_k_f64(const mp_ctx* c, int W, int requested, int form) {
     auto form_of = [&](int k) {
         if (k != 8) return 0;
         if (form > 0) return form;
-        return items_of(8) > S ? 2 : 1;
+        return items_of(8) > S ? 3 : 1;
     };
     if (requested > 0) return WalkChoice{requested, form_of(requested)};
     static const double c1[4] = {2.19, 1.30, 0.75, 0.38};
@@ -140,7 +141,7 @@ WalkChoice pick_band_k_f64(const mp_ctx* c, int W, int requested, int form) {
         const int occ = (k == 8) ? (pair ? 2 : 1) : 4;
         const int64_t s = std::max<int64_t>(1, std::min<int64_t>(occ, (items + S - 1) / S));
         const int64_t rounds = (items + S * s - 1) / (S * s);
-        const double per_wave = pair ? 0.637 : c1[i] * static_cast<double>(s) * g[s];
+        const double per_wave = pair ? 0.608 : c1[i] * static_cast<double>(s) * g[s];
         const double t = static_cast<double>(rounds) * 64.0 * k * per_wave;
         if (i == 0 || t < best_t) {
             best_t = t;
@@ -283,7 +284,7 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
     const int m = g.m;
     const int64_t index_init = (o && o->index_init == 0) ? 0 : -1;
     const int W = g.W;
-    if (o && (o->walk_form < 0 || o->walk_form > 2)) return MP_ERR_BAD_ARG;
+    if (o && (o->walk_form < 0 || o->walk_form > 3)) return MP_ERR_BAD_ARG;
     const WalkChoice wc = pick_band_k_f64(c, W, o ? o->band_k : 0, o ? o->walk_form : 0);
     const int band_k = wc.band_k;
     if (band_k != 1 && band_k != 2 && band_k != 4 && band_k != 8) return MP_ERR_BAD_ARG;
@@ -335,7 +336,7 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
         const int nb = mp::walk_f64_nbands(W, band_k);
         const int items = (nb + 1) / 2;
         a.form = wc.form;
-        a.groups = std::max(1, std::min((items + 3) / 4, (a.form == 2 ? 2 : 8) * c->cus));
+        a.groups = std::max(1, std::min((items + 3) / 4, ((a.form == 2 || a.form == 3) ? 2 : 8) * c->cus));
         MP_TRY(mp::launch_walk_f64(a, st));
     }
     MP_HIP(hipEventRecord(c->ev[3], st));

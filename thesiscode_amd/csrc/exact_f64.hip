@@ -759,7 +759,300 @@ __device__ __forceinline__ void walk_body(
     }
 }
 
-#define MP_WALK_PARAMS                                                                         \
+// Lane l receives lane l-1's value, lane 0 lane 63's (DPP wave_ror:1).
+// Every lane has a source lane, so no lane keeps an old value.
+__device__ __forceinline__ double rotate_up_f64(double src) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(src), 0x13C, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(src), 0x13C, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Rotating band walker: the rows stay in place and the diagonals move.  The
+// band's diagonals, cells, record streams, masking and result merging are
+// those of walk_body; what differs is which lane holds what.  Band row b (row
+// dbase + b) sits at band position b mod BW, in lane (b mod BW) / K, slot
+// b mod K, for its whole time in the band: at column j the wave holds rows
+// j .. j+BW-1, and the row that has its last cell at column j (b = j, diagonal
+// dbase) is replaced in place by row j+BW when the step ends.  Diagonal
+// dbase + e is at position (j + e) mod BW at column j, so each QT value moves
+// one position per step: within a lane by the choice of register (register t
+// holds the diagonal at slot (t + j) mod K), and from lane l's last slot to
+// lane l+1's first (lane 63's to lane 0's) by one DPP rotation of one double.
+// A step thus moves 2 dwords between lanes where walk_body moves a row record,
+// its running maximum and its lane tag.
+//
+// Rows.  A row's maximum never leaves its lane: slot s keeps the maximum of the
+// current block of K steps (lm) and of the row's finished blocks (run), and
+// the last column of the block that holds run (tag: the first such block, as
+// blocks end in column order and only a strictly larger maximum replaces it).
+// Blocks of slot s end at the columns j = s (mod K); the leaving row's block
+// ends at its last column, where its lane alone (exec-masked LDS operations)
+// writes (run, tag) for the flush and takes the entering row: run becomes a
+// quiet NaN (its high word set), which every block maximum replaces, and the
+// row's record comes from the LDS staging of FEEDLDS.  A row result's key
+// names the K columns of the block, tag - k, and the unpack picks among them.
+//
+// Columns.  At column j the lane holding position j mod BW (lane wl, wrap slot
+// S = j mod K) holds rows j .. j+K-1-S in slots S .. K-1 and the band's last
+// rows j+BW-S .. j+BW-1 in slots 0 .. S-1; every other lane holds K
+// consecutive rows, those of lane wl + r starting at j + K r - S.  Every lane
+// writes the pair (maximum over slots S .. K-1, maximum over slots 0 .. S-1)
+// at index r = (l - wl) mod 64, so the pairs of a column lie in ascending row
+// order.  The flush takes pair r's maximum for r > 0, the first half of pair 0
+// as partial 0 and its second half as a 65th partial (index 64, the largest
+// rows): its smallest index holding the maximum is then the smallest row, as
+// the keys rank exact ties, and the key names the K rows from that partial's
+// first (index 64: the K rows ending at j+BW-1).
+//
+// A step has one exec-masked region (each costs a branch): the column pairs
+// are written by every lane.
+template <int K>
+__device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec,
+                                              const WalkRec* __restrict__ rowrec, const double* qt,
+                                              unsigned long long* rowval, unsigned long long* rowkey,
+                                              unsigned long long* colval, unsigned long long* colkey,
+                                              unsigned int* counter, int64_t row_off, int64_t col_off,
+                                              int blocklen, int tl, int ib, int nbands) {
+    constexpr int F = kFlush;
+    constexpr int BW = kWave * K;
+    // a window starts at wrap slot 0 and its positions never pass the wave's end
+    static_assert(F % K == 0 && BW % F == 0, "window and band geometry");
+    constexpr int Q = kWave / 4;  // pairs per flush lane
+    struct RowRes {
+        double v;
+        int tag, pad;
+    };
+    // column pairs, rows padded by one pair: the 16 flush lanes reading one
+    // index of 16 columns then fall on distinct banks
+    __shared__ double2 s_cv[kWavesPerGroup][F][kWave + 1];
+    __shared__ RowRes s_rr[kWavesPerGroup][F];  // the leaving row's result per step
+    __shared__ __align__(16) WalkRec s_feed[kWavesPerGroup][2 * F];
+
+    const int wv = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const double NEG = -__builtin_inf();
+    const double QNAN = __builtin_nan("");
+    const int nitems = (nbands + 1) / 2;
+    const int lid0 = lane * K;
+    WalkRec* const feed = &s_feed[wv][0];
+
+    // Flush columns [jstart, jstart+cnt) and the rows that left at those
+    // columns, as walk_body's flush does; lane (cs, qq) reduces a quarter of
+    // column cs's partials and the 65th.
+    auto flush = [&](int jstart, int cnt, int dbase, uint64_t& pc, uint64_t& pr, auto reload)
+        __attribute__((always_inline)) {
+        __builtin_amdgcn_wave_barrier();
+        const int cs = lane & (F - 1), qq = lane / F;
+        const double2* const pv = &s_cv[wv][cs][qq * Q];
+        double v[Q + 1];
+#pragma unroll
+        for (int u = 0; u < Q; ++u) {
+            const double2 pr2 = pv[u];
+            v[u] = dmax(pr2.x, pr2.y);
+            if (u == 0) v[0] = (qq == 0) ? pr2.x : v[0];
+        }
+        v[Q] = s_cv[wv][cs][0].y;
+        const double thr = unord_f64((pc >> ib) << ib);
+        double mx = max_tree<Q + 1>(v);
+        const bool cand = (mx >= thr) && (cs < cnt);
+        const int jj = jstart + lane;
+        uint64_t oc = 0;
+        ScoreId b{NEG, -1};
+        if (__builtin_amdgcn_ballot_w64(cand) != 0) {
+            mx = max_quarters(mx);
+            int id = (v[Q] == mx) ? kWave : kWave + 1;
+#pragma unroll
+            for (int u = Q - 1; u >= 0; --u) id = (v[u] == mx) ? qq * Q + u : id;
+            id = min_quarters(id);
+            b = ScoreId{mx, id <= kWave ? id : -1};
+            oc = ord_f64(mx);
+        }
+        const bool do_col = (lane < cnt) & (b.id >= 0) & (b.v == b.v) & ((oc >> ib) >= (pc >> ib));
+        const RowRes rr = s_rr[wv][lane & (F - 1)];
+        const uint64_t orw = ord_f64(rr.v);
+        // a row whose cells were all masked keeps -inf or the reset NaN
+        const bool do_row = (lane < cnt) & (rr.v > NEG) & ((orw >> ib) >= (pr >> ib));
+        __builtin_amdgcn_sched_barrier(0);
+        reload(pc, pr);
+        if (do_col) {
+            // first row of the winning partial; index kWave: the K rows ending at jj+BW-1
+            const int S = jj & (K - 1);
+            const bool last = b.id == kWave;
+            const int64_t b0 = last ? static_cast<int64_t>(jj) + BW - 1
+                                    : static_cast<int64_t>(jj) + max(K * b.id - S, 0);
+            const int64_t ccode = ((b0 + dbase + col_off) << 1) | (last ? 1 : 0);
+            atomicMax(&colval[jj], static_cast<unsigned long long>(oc));
+            atomicMax(&colkey[jj], static_cast<unsigned long long>(make_key(oc, ib, ccode)));
+        }
+        if (do_row) {
+            const int i = jj + dbase;
+            const int64_t rcode = ((static_cast<int64_t>(rr.tag) + row_off) << 1) | 1;
+            atomicMax(&rowval[i], static_cast<unsigned long long>(orw));
+            atomicMax(&rowkey[i], static_cast<unsigned long long>(make_key(orw, ib, rcode)));
+        }
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    for (;;) {
+        const int item = next_item(counter, lane);
+        if (item >= nitems) break;
+        for (int h = 0; h < 2; ++h) {
+            const int band = (h == 0) ? item : nbands - 1 - item;
+            if (h == 1 && band == item) break;
+            const int dbase = band * BW;
+            const int L = tl - dbase;
+            double qr[K], wa[K], wam[K], ws[K], wm[K], lm[K], run[K];
+            int tag[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int d = dbase + lid0 + k;  // at column 0 position e holds diagonal and row e
+                qr[k] = (d < blocklen) ? qt[d] : QNAN;
+                const WalkRec r = rowrec[d];
+                wa[k] = r.a;
+                wam[k] = r.am;
+                ws[k] = r.s;
+                wm[k] = r.mu;
+                lm[k] = NEG;
+                run[k] = NEG;
+                tag[k] = -1;
+            }
+            WalkRec cur_c;             // the next step's column record
+            const WalkRec* fw = feed;  // this window's entering rows
+            int wl0 = 0;               // the wrap lane of the window's first column
+            // this lane's rotated index, per half window (its wrap lane)
+            int rot = 0;
+            auto half = [&](int wl) __attribute__((always_inline)) { rot = (lane - wl) & (kWave - 1); };
+            // A step's column pair is written by the next step, after its wait
+            // for the scalar records (as in walk_body).
+            double pav = NEG, pbv = NEG;
+            auto put_pair = [&](int w, int r) __attribute__((always_inline)) {
+                s_cv[wv][w][r] = make_double2(pav, pbv);
+            };
+            auto step = [&](int c0, auto w_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
+                constexpr int W = decltype(w_tag)::value;
+                constexpr int S = W % K;             // the wrap slot: its row leaves after this step
+                constexpr int S0 = (W + K - 1) % K;  // the slot whose block starts at this step
+                constexpr bool TAIL = decltype(tail_tag)::value;
+                const int j = c0 + W;
+                if (TAIL && j >= L) return false;
+                __builtin_amdgcn_sched_barrier(0);
+                // Scalar loads may return out of order, so the wait for this
+                // step's record (forced by the empty asm, which needs the
+                // values) covers every scalar and LDS operation in flight; the
+                // next step's record is loaded after it.
+                const WalkRec c = cur_c;
+                asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu) : "memory");
+                cur_c = colrec[j + 1];
+                if constexpr (W > 0) {
+                    const int rprev = rot;  // the last step's wrap lane
+                    if constexpr (S == 0) half(wl0 + W / K);
+                    put_pair(W - 1, rprev);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                const double ahj = c.a, ahjm = c.am, shj = c.s, mhj = c.mu;
+                double sc[K];
+                auto cell = [&](auto s_tag) __attribute__((always_inline)) {
+                    constexpr int s = decltype(s_tag)::value;
+                    constexpr int t = (s - S + K) % K;  // the register of slot s's diagonal
+                    sc[s] = fma(qr[t], ws[s] * shj, -(wm[s] * mhj));
+                    qr[t] = fma(wam[s], ahjm, qr[t]);
+                    qr[t] = fma(-wa[s], ahj, qr[t]);
+                    lm[s] = (s == S0) ? sc[s] : dmax(lm[s], sc[s]);
+                };
+                // the leaving row first, so that its successor's record loads
+                // while the other slots compute
+                cell(std::integral_constant<int, S>{});
+                // The QT updates read the rows: the compiler would sink them to
+                // their next use, the next step, past the masked region that
+                // replaces a row, and copy that row's registers to keep them.
+                // Empty asm statements that take the updated values keep them
+                // here (the leaving slot's) and in the step (the others).
+                asm volatile("" ::"v"(qr[0]));
+                tag[S] = !(lm[S] <= run[S]) ? j : tag[S];
+                run[S] = dmax(run[S], lm[S]);
+                if (__builtin_expect(rot == 0, 1)) {
+                    s_rr[wv][W].v = run[S];
+                    s_rr[wv][W].tag = tag[S];
+                    run[S] = __hiloint2double(static_cast<int>(0xFFF80000u), __double2loint(run[S]));
+                    const double2* p = reinterpret_cast<const double2*>(fw + W);
+                    const double2 lo = p[0], hi = p[1];
+                    wa[S] = lo.x;
+                    wam[S] = lo.y;
+                    ws[S] = hi.x;
+                    wm[S] = hi.y;
+                }
+                [&]<int... s>(std::integer_sequence<int, s...>) __attribute__((always_inline)) {
+                    ((s != S ? cell(std::integral_constant<int, s>{}) : void()), ...);
+                }(std::make_integer_sequence<int, K>{});
+                // slot K-1's diagonal moves to the next lane's slot 0
+                constexpr int tl7 = (K - 1 - S + K) % K;
+                qr[tl7] = rotate_up_f64(qr[tl7]);
+#pragma unroll
+                for (int t = 1; t < K; ++t) asm volatile("" ::"v"(qr[t]));
+                pav = max_tree<K - S, S>(sc);
+                pbv = (S == 0) ? NEG : max_tree<(S > 0 ? S : 1), 0>(sc);
+                return true;
+            };
+            // rows dbase + c0 + BW + x, x < F (padding records past the
+            // triangle), to buffer nb: lane l < 2F loads half l % 2 of row l / 2
+            auto stage = [&](int c0, int nb) __attribute__((always_inline)) {
+                if (lane < 2 * F) {
+                    const char* src =
+                        reinterpret_cast<const char*>(rowrec + (dbase + c0 + BW + (lane >> 1))) + 16 * (lane & 1);
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
+                                                     (__attribute__((address_space(3))) void*)(feed + nb * F), 16,
+                                                     0, 0);
+                }
+            };
+            auto window = [&](int c0, auto tail_tag) __attribute__((always_inline)) {
+                // the next window's entering rows (band rows c0 + F + BW + x) to
+                // the other buffer, 16 bytes per lane; this window's landed
+                // before the last flush
+                __builtin_amdgcn_wave_barrier();
+                const int nb = (c0 / F) & 1;
+                stage(c0 + F, nb ^ 1);
+                fw = feed + nb * F;
+                wl0 = (c0 % BW) / K;
+                half(wl0);
+                __builtin_amdgcn_wave_barrier();
+                bool go = true;
+                [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
+                    ((go = go && step(c0, std::integral_constant<int, W>{}, tail_tag)), ...);
+                }(std::make_integer_sequence<int, F>{});
+                const int wlast = decltype(tail_tag)::value ? min(F, L - c0) - 1 : F - 1;
+                put_pair(wlast, rot);
+            };
+            auto current = [&](int c0, uint64_t& pc, uint64_t& pr) __attribute__((always_inline)) {
+                const int jj = min(c0 + (lane & (F - 1)), L - 1);
+                pc = colval[jj];
+                pr = rowval[jj + dbase];
+            };
+            int c0 = 0;
+            uint64_t pc = 0, pr = 0;
+            current(0, pc, pr);
+            // the first window's entering rows (band rows BW + x) to buffer 0
+            __builtin_amdgcn_wave_barrier();
+            stage(0, 0);
+            __builtin_amdgcn_wave_barrier();
+            cur_c = colrec[0];
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            for (; c0 + F <= L; c0 += F) {
+                window(c0, std::false_type{});
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
+                    current(c0 + F, a, b);
+                });
+            }
+            if (c0 < L) {
+                window(c0, std::true_type{});
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
+            }
+        }
+    }
+}
+
+#define MP_WALK_PARAMS                                                                       \
     const WalkRec *__restrict__ colrec, const WalkRec *__restrict__ rowrec, double *qt,          \
         unsigned long long *rowval, unsigned long long *rowkey, unsigned long long *colval,     \
         unsigned long long *colkey, unsigned int *counter, int64_t row_off, int64_t col_off,    \
@@ -790,6 +1083,15 @@ template <int K, bool WRITEBACK>
 __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(2))) void k_walk_f64_pair(
     MP_WALK_PARAMS) {
     walk_body<K, WRITEBACK, false, false>(MP_WALK_ARGS);
+}
+// K = 8 with rows in place and the diagonals rotating (walk_rot_body); the
+// self-join only (no final-QT write-back).
+template <int K>
+__global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(2))) void k_walk_f64_rot(
+    MP_WALK_PARAMS) {
+    (void)m;
+    walk_rot_body<K>(colrec, rowrec, qt, rowval, rowkey, colval, colkey, counter, row_off, col_off, blocklen,
+                     tl, ib, nbands);
 }
 #undef MP_WALK_PARAMS
 #undef MP_WALK_ARGS
@@ -944,7 +1246,12 @@ int launch_walk_f64(const WalkF64Launch& a, hipStream_t st) {
                                a.qt, a.rowval, a.rowkey, a.colval, a.colkey, a.counter,          \
                                a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib, nbands);       \
         break;
-    if (a.band_k == 8 && a.form == 2) {
+    if (a.band_k == 8 && a.form == 3) {
+        if (a.writeback) return MP_ERR_BAD_ARG;
+        hipLaunchKernelGGL((k_walk_f64_rot<8>), grid, block, 0, st, a.colrec, a.rowrec, a.qt, a.rowval, a.rowkey,
+                           a.colval, a.colkey, a.counter, a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib,
+                           nbands);
+    } else if (a.band_k == 8 && a.form == 2) {
         switch (a.band_k) { MP_WALK_CASE(8, k_walk_f64_pair) }
     } else {
         switch (a.band_k) {

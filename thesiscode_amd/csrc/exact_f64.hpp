@@ -74,7 +74,8 @@ struct WalkF64Launch {
     int blocklen, tl, m, ib, writeback, band_k, groups;
     int form;               // band_k = 8: 1 rows from an LDS ring, one wave per SIMD
                             // (k_walk_f64_wide); 2 rows by DPP, two waves per SIMD
-                            // (k_walk_f64_pair)
+                            // (k_walk_f64_pair); 3 rows in place, diagonals rotating
+                            // by DPP, two waves per SIMD (k_walk_f64_rot, no write-back)
 };
 int walk_f64_nbands(int blocklen, int band_k);
 int launch_walk_f64(const WalkF64Launch& a, hipStream_t st);
