@@ -149,8 +149,9 @@ __global__ void k_fill_u32(unsigned int* __restrict__ p, int count, unsigned int
 // diagonals (code = p0 << 1 | dir: p0 + k, or p0 - k when dir = 1); the one
 // whose recomputed score is closest to the exact maximum is the index (equal
 // distances: the smaller partner).  The candidates' windows overlap, so each
-// block of K window positions loads K values of window i and 2K-1 of the
-// candidates' and does K*K FMAs.  Candidates outside the profile or inside the
+// block of K window positions loads K values of window i and K of the
+// candidates' (their other K-1 values are the previous block's last ones) and
+// does K*K FMAs.  Candidates outside the profile or inside the
 // exclusion zone are not cells and are skipped.
 template <int K>
 __device__ __forceinline__ int64_t pick_candidate(const double* __restrict__ A, const double* __restrict__ sInv,
@@ -164,18 +165,25 @@ __device__ __forceinline__ int64_t pick_candidate(const double* __restrict__ A, 
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    // ys slides along y: a block loads its last K values, the first K-1 are
+    // the previous block's last ones
+    double ys[2 * K - 1];
+#pragma unroll
+    for (int v = 0; v < K - 1; ++v) ys[v] = y[v];
     int t = 0;
     for (; t + K <= m; t += K) {
-        double xs[K], ys[2 * K - 1];
+        double xs[K];
 #pragma unroll
         for (int u = 0; u < K; ++u) xs[u] = x[t + u];
 #pragma unroll
-        for (int v = 0; v < 2 * K - 1; ++v) ys[v] = y[t + v];
+        for (int v = K - 1; v < 2 * K - 1; ++v) ys[v] = y[t + v];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
 #pragma unroll
             for (int u = 0; u < K; ++u) acc[k] = fma(xs[u], ys[u + k], acc[k]);
         }
+#pragma unroll
+        for (int v = 0; v < K - 1; ++v) ys[v] = ys[v + K];
     }
     for (; t < m; ++t) {
 #pragma unroll

@@ -660,9 +660,10 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32_wide(MP_TILE_PARAMS) {
 // One wave per chunk of 64 consecutive entries, one entry per lane: the series
 // of the chunk's first entry is found once, and a lane whose entry lies in a
 // later series advances from there.  The candidates' windows overlap: a lane
-// walks its window x_t = A[i+t] - mu_i in blocks of K positions, loads the
+// walks its window x_t = A[i+t] - mu_i in blocks of K positions, holds the
 // 2K-1 values y = A[lo+t+v] - mu_lo that the K candidates lo .. lo+K-1 need
-// there and does K*K FMAs.  Loads past the last candidate's window (kcount <
+// there (K of them loaded, K-1 kept from the previous block) and does K*K
+// FMAs.  Loads past the last candidate's window (kcount <
 // K near the end of a series) are skipped, so no read leaves the series.
 constexpr int kUnpackChunk = 64;
 
@@ -714,21 +715,28 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
                 double acc[K], sx = 0.0;
 #pragma unroll
                 for (int k = 0; k < K; ++k) acc[k] = 0.0;
+                // ys slides along y: a block loads its last K values, the first
+                // K-1 are the previous block's last ones
+                double ys[2 * K - 1];
+#pragma unroll
+                for (int v = 0; v < K - 1; ++v) ys[v] = (v < ylim) ? y[v] - ml : 0.0;
                 int t = 0;
                 for (; t + K <= m; t += K) {
-                    double xs[K], ys[2 * K - 1];
+                    double xs[K];
 #pragma unroll
                     for (int u = 0; u < K; ++u) {
                         xs[u] = x[t + u] - mi;
                         sx += xs[u];
                     }
 #pragma unroll
-                    for (int v = 0; v < 2 * K - 1; ++v) ys[v] = (t + v < ylim) ? y[t + v] - ml : 0.0;
+                    for (int v = K - 1; v < 2 * K - 1; ++v) ys[v] = (t + v < ylim) ? y[t + v] - ml : 0.0;
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
 #pragma unroll
                         for (int u = 0; u < K; ++u) acc[k] = fma(xs[u], ys[u + k], acc[k]);
                     }
+#pragma unroll
+                    for (int v = 0; v < K - 1; ++v) ys[v] = ys[v + K];
                 }
                 for (; t < m; ++t) {
                     const double xv = x[t] - mi;
