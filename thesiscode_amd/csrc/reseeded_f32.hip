@@ -666,6 +666,11 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32_wide(MP_TILE_PARAMS) {
 // FMAs.  Loads past the last candidate's window (kcount <
 // K near the end of a series) are skipped, so no read leaves the series.
 constexpr int kUnpackChunk = 64;
+// Two adjacent doubles read by one 16-byte load (8-byte alignment suffices on
+// gfx950, whose global loads need not be naturally aligned).
+struct alignas(8) DoublePair {
+    double a, b;
+};
 
 template <int K>
 __global__ __launch_bounds__(256) void k_unpack_f32r(
@@ -721,15 +726,30 @@ __global__ __launch_bounds__(256) void k_unpack_f32r(
 #pragma unroll
                 for (int v = 0; v < K - 1; ++v) ys[v] = (v < ylim) ? y[v] - ml : 0.0;
                 int t = 0;
+                // with all K candidates every read of a block lies within the
+                // series (t + 2K - 2 < m + K - 1): two values per 16-byte load
+                const bool full = kcount == K;
                 for (; t + K <= m; t += K) {
                     double xs[K];
 #pragma unroll
-                    for (int u = 0; u < K; ++u) {
-                        xs[u] = x[t + u] - mi;
-                        sx += xs[u];
+                    for (int u = 0; u < K; u += 2) {
+                        const DoublePair q = *reinterpret_cast<const DoublePair*>(x + t + u);
+                        xs[u] = q.a - mi;
+                        xs[u + 1] = q.b - mi;
                     }
 #pragma unroll
-                    for (int v = K - 1; v < 2 * K - 1; ++v) ys[v] = (t + v < ylim) ? y[t + v] - ml : 0.0;
+                    for (int u = 0; u < K; ++u) sx += xs[u];
+                    if (full) {
+#pragma unroll
+                        for (int v = K - 1; v < 2 * K - 1; v += 2) {
+                            const DoublePair q = *reinterpret_cast<const DoublePair*>(y + t + v);
+                            ys[v] = q.a - ml;
+                            ys[v + 1] = q.b - ml;
+                        }
+                    } else {
+#pragma unroll
+                        for (int v = K - 1; v < 2 * K - 1; ++v) ys[v] = (t + v < ylim) ? y[t + v] - ml : 0.0;
+                    }
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
 #pragma unroll
