@@ -99,6 +99,7 @@ def run_gpu(workload: str, steps: int, warmup: int, band_k: int = 0, tile_len: i
             "value": tm["cells"] * steps / wall,
             "kernel_ms": float(np.mean(kern_ms)), "device_ms": float(np.mean(total_ms)),
             "band_k": tm["band_k"], "tile_len": tm["tile_len"], "work_items": tm["work_items"],
+            "kernel": kernel_name(prec, tm, accum64),
         }
         if download:
             out["profile"] = dev.download(d_prof, np.empty(P))
@@ -109,6 +110,15 @@ def run_gpu(workload: str, steps: int, warmup: int, band_k: int = 0, tile_len: i
     finally:
         if own:
             dev.close()
+
+
+def kernel_name(precision: str, tm: dict, accum64: bool = True) -> str:
+    """The dominant kernel of a self-join, as rocprof names it."""
+    k = tm["band_k"]
+    if precision == "f32r":
+        return f"k_tiles_f32<{k},{'true' if accum64 else 'false'}>" if k <= 8 else f"k_tiles_f32_wide<{k}>"
+    form = {1: "k_walk_f64_wide", 2: "k_walk_f64_pair", 3: "k_walk_f64_rot"}.get(tm.get("walk_form", 0))
+    return f"{form}<{k}>" if (k == 8 and form) else f"k_walk_f64<{k}>"
 
 
 def profile_error(p, i, p64, i64, m):
@@ -475,7 +485,8 @@ def default_run(args) -> dict:
                             precision="f32r", download=True)
                 err = profile_error(v.pop("profile"), v.pop("index"), p64, i64, m)
                 name = f"{workload.split('-')[0]}-f32r-B{v['tile_len']}" + ("-default" if B == 0 else "")
-                variants[name] = {"value": v["value"], "steps": v["steps"], "kernel_ms": v["kernel_ms"],
+                variants[name] = {"value": v["value"], "steps": v["steps"], "kernel": v["kernel"],
+                                  "kernel_ms": v["kernel_ms"],
                                   "device_ms": v["device_ms"], "band_k": v["band_k"], "tile_len": v["tile_len"],
                                   "accum64": True, **err,
                                   "frac_fp32_peak": FLOPS_PER_CELL * v["cells"] / (v["kernel_ms"] / 1e3) / 1e12
@@ -590,7 +601,7 @@ def main() -> int:
         "config": {"workload": args.workload, "n": r["n"], "m": r["m"], "exclusion": r["m"] // 4,
                    "cells": r["cells"], "precision": r["precision"], "band_k": r["band_k"],
                    "tile_len": r["tile_len"], "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "kernel": "k_walk_f64" if f64 else "k_tiles_f32", "achieved": achieved,
+        "roofline": {"bound": "valu", "kernel": r["kernel"], "achieved": achieved,
                      "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": read_traffic(args.workload), "kernel_ms": r["kernel_ms"],
                      "flops_per_cell": FLOPS_PER_CELL},
@@ -600,7 +611,7 @@ def main() -> int:
         out["variants"] = legs["variants"]
         dflt = next(v for k, v in legs["variants"].items() if k.endswith("-default"))
         a32 = FLOPS_PER_CELL * r["cells"] / (dflt["kernel_ms"] / 1000.0) / 1e12
-        out["roofline_f32r"] = {"bound": "valu", "kernel": "k_tiles_f32", "achieved": a32,
+        out["roofline_f32r"] = {"bound": "valu", "kernel": dflt["kernel"], "achieved": a32,
                                 "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                                 "frac": a32 / FP32_VECTOR_PEAK_TFLOPS,
                                 "traffic": read_traffic(args.workload.split("-")[0] + "-f32r"),

@@ -76,6 +76,9 @@ typedef struct mp_timing {
     double cells;        /* cells evaluated: W(W+1)/2, W = P - excl - 1                     */
     int64_t work_items;  /* bands (exact path) or tiles (FP32 path) processed              */
     int32_t band_k, tile_len;
+    int32_t walk_form;   /* exact path at band width 8: the walker form that ran (mp_opts.walk_form
+                            numbering: 1 LDS ring, 2 DPP rows, 3 rotating); 0 otherwise */
+    int32_t reserved;
 } mp_timing;
 
 /* ---- device context (one per thread or stream; calls on one are serialised) */

@@ -72,6 +72,7 @@ def test_band_width_8_walker_forms_bit_exact(walk_form, n, m, seed):
     print(f"form {walk_form} n={n} m={m}", flush=True)
     p, i, tm = tc.selfjoin_f64(A, m, band_k=8, walk_form=walk_form)
     assert tm["band_k"] == 8
+    assert tm["walk_form"] == walk_form
     assert np.array_equal(p, p_ref)
     assert_indices_match(A, m, p_ref, i_ref, i, abs_tol=_tie_tol(m), min_frac=0.999)
 

@@ -36,6 +36,7 @@ class Timing(C.Structure):
         ("total_ms", C.c_double), ("stats_ms", C.c_double), ("seeds_ms", C.c_double),
         ("kernel_ms", C.c_double), ("finalize_ms", C.c_double), ("cells", C.c_double),
         ("work_items", C.c_int64), ("band_k", C.c_int32), ("tile_len", C.c_int32),
+        ("walk_form", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -355,6 +355,7 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
         tm->work_items = W > 0 ? mp::walk_f64_nbands(W, band_k) : 0;
         tm->band_k = band_k;
         tm->tile_len = 0;
+        tm->walk_form = (band_k == 8) ? wc.form : 0;
     }
     return MP_OK;
 }
