@@ -35,6 +35,12 @@
 
 namespace mp {
 
+// Two adjacent doubles read by one 16-byte load (8-byte alignment suffices on
+// gfx950, whose global loads need not be naturally aligned).
+struct alignas(8) DoublePair {
+    double a, b;
+};
+
 __device__ __forceinline__ void neu_add(double& s, double& c, double x) {
     const double t = s + x;
     if (fabs(s) >= fabs(x))
@@ -74,7 +80,12 @@ __device__ __forceinline__ int find_series(const int64_t* __restrict__ prefix, i
 // One thread per chunk of kStatChunk windows of one series: a direct
 // compensated sum at the chunk's first window, then sliding updates.  Chunks
 // start at series-local multiples of kStatChunk, so a series gets the same
-// statistics alone or in any batch.
+// statistics alone or in any batch.  A lane's values arrive in blocks of 8
+// (four 16-byte loads), the next block's loads issued before the current
+// block's sums, so the dependent sums do not wait on each load in turn.  A
+// block can reach past the series (at most 14 elements; their values are not
+// used): every series is followed by the next one or by the 64 elements of
+// padding its buffer has, and reads are clamped to that padding.
 __global__ void k_comp_stats(const SeriesDesc* __restrict__ descs,
                              const int64_t* __restrict__ stat_prefix, int nseries, int64_t nchunks,
                              const double* __restrict__ A, double* __restrict__ mu,
@@ -88,22 +99,37 @@ __global__ void k_comp_stats(const SeriesDesc* __restrict__ descs,
     const double* a = A + d.off;
     const int m = d.m;
     const double dm = static_cast<double>(m);
+    const int lim = d.n + 56;  // a pair read from here stays inside the series' 64-element padding
+    constexpr int B = 8;
+    auto load = [&](int at, double (&v)[B]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < B; u += 2) {
+            const DoublePair q = *reinterpret_cast<const DoublePair*>(a + min(at + u, lim));
+            v[u] = q.a;
+            v[u + 1] = q.b;
+        }
+    };
     // data shifted by the chunk's first value: sums of small numbers
     const double shift = a[i0];
     double S = 0.0, Sc = 0.0, Q = 0.0, Qc = 0.0;
-    for (int k = 0; k < m; ++k) {
-        const double x = a[i0 + k] - shift;
-        neu_add(S, Sc, x);
-        neu_add_sq(Q, Qc, x);
-    }
-    for (int i = i0; i < i1; ++i) {
-        if (i > i0) {  // window i-1 -> window i
-            const double xo = a[i - 1] - shift, xn = a[i - 1 + m] - shift;
-            neu_add(S, Sc, xn);
-            neu_add(S, Sc, -xo);
-            neu_add_sq(Q, Qc, xn);
-            neu_sub_sq(Q, Qc, xo);
+    {
+        double v[B], w[B];
+        load(i0, v);
+        for (int k0 = 0; k0 < m; k0 += B) {
+            load(i0 + k0 + B, w);
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                if (k0 + u < m) {
+                    const double x = v[u] - shift;
+                    neu_add(S, Sc, x);
+                    neu_add_sq(Q, Qc, x);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < B; ++u) v[u] = w[u];
         }
+    }
+    auto emit = [&](int i) __attribute__((always_inline)) {
         const double sm = (S + Sc) / dm;
         const double qm = (Q + Qc) / dm;
         double var = fma(-sm, sm, qm);
@@ -112,6 +138,32 @@ __global__ void k_comp_stats(const SeriesDesc* __restrict__ descs,
         if (!(var > 64.0 * __DBL_EPSILON__ * qm)) var = 0.0;
         mu[d.off + i] = shift + sm;
         sigma[d.off + i] = sqrt(var);
+    };
+    emit(i0);
+    // window i0 + 1 + q drops a[i0 + q] and takes a[i0 + m + q]
+    const int nq = i1 - i0 - 1;
+    double vo[B], vn[B], wo[B], wn[B];
+    load(i0, vo);
+    load(i0 + m, vn);
+    for (int q0 = 0; q0 < nq; q0 += B) {
+        load(i0 + q0 + B, wo);
+        load(i0 + m + q0 + B, wn);
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            if (q0 + u < nq) {
+                const double xo = vo[u] - shift, xn = vn[u] - shift;
+                neu_add(S, Sc, xn);
+                neu_add(S, Sc, -xo);
+                neu_add_sq(Q, Qc, xn);
+                neu_sub_sq(Q, Qc, xo);
+                emit(i0 + 1 + q0 + u);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            vo[u] = wo[u];
+            vn[u] = wn[u];
+        }
     }
 }
 
@@ -666,11 +718,6 @@ __global__ __launch_bounds__(kGroup) void k_tiles_f32_wide(MP_TILE_PARAMS) {
 // FMAs.  Loads past the last candidate's window (kcount <
 // K near the end of a series) are skipped, so no read leaves the series.
 constexpr int kUnpackChunk = 64;
-// Two adjacent doubles read by one 16-byte load (8-byte alignment suffices on
-// gfx950, whose global loads need not be naturally aligned).
-struct alignas(8) DoublePair {
-    double a, b;
-};
 
 template <int K>
 __global__ __launch_bounds__(256) void k_unpack_f32r(
