@@ -175,6 +175,33 @@ def test_config5_poisson_trace():
         print(f"request {k}: n={ts.size} m={m} max rel vs long double {worst:.2e}", flush=True)
 
 
+def test_small_requests_are_not_held_behind_a_large_one():
+    # Continuous batching, not a queue: a 2^19-point request (about 1.4e11 cells)
+    # is submitted first and 16 small ones right after it; every tick gives each
+    # active request a share of its tiles, so the small ones finish while the
+    # large one still runs, and each result equals its own single run.
+    big = (tc.random_walk(91, 1 << 19), 64)
+    small = [(tc.random_walk(92 + k, 1 << 12), 64) for k in range(16)]
+    with tc.Device(0) as dev:
+        with tc.Server(dev, hbm_budget_bytes=1 << 30) as srv:
+            big_id = srv.submit(*big)
+            small_ids = [srv.submit(ts, m) for ts, m in small]
+            order, res = [], {}
+            t0 = time.time()
+            while len(order) < 17 and time.time() - t0 < 60:
+                for rid, status, lat, q, p, i in srv.completed():
+                    assert status == 0
+                    order.append(rid)
+                    res[rid] = (p, i, lat)
+                time.sleep(0.0005)
+    assert len(order) == 17
+    assert order[-1] == big_id, "a small request finished after the large one"
+    assert max(res[r][2] for r in small_ids) < res[big_id][2]
+    for (ts, m), rid in [(big, big_id), (small[0], small_ids[0]), (small[15], small_ids[15])]:
+        p1, i1, _ = tc.selfjoin_f32r(ts, m, band_k=8, tile_len=4096)
+        assert np.array_equal(res[rid][0], p1) and np.array_equal(res[rid][1], i1)
+
+
 def test_wait_and_completed_on_two_threads():
     # mp_server_wait on one thread while another drains mp_server_completed:
     # every request is handed out (to the waiter, or to the drain when it took
