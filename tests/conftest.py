@@ -72,3 +72,39 @@ def assert_indices_match(A, m, prof_ref, idx_ref, idx_got, *, abs_tol, min_frac=
         d_got = orc.pair_dist2(A, m, i, idx_got[i])
         tol = abs_tol + rel_tol * max(d_ref, 2 * m * 1e-3)
         assert abs(d_ref - d_got) <= tol, (i, idx_ref[i], idx_got[i], d_ref, d_got, prof_ref[i])
+
+
+def assert_values_equal_reference_walk(A, m, prof, idx, entries, *, excl=-1):
+    """Exact-path values at `entries` equal the reference's arithmetic bit for bit.
+
+    For each entry the oracle walks the 15 diagonals around the entry's partner
+    from their first cells (the reference's seeds and recurrence,
+    ScrimpSequOpt.cpp:81-96, kernels.cpp:310-366) through the separate-view
+    adapter (row results by row, column results by column).  Those cells, on
+    the partner's side of the entry, include the winning lane's candidates (at
+    most 8 adjacent diagonals), so their maximum score is the profile value:
+    prof[e] == 2m - 2 * max.
+    """
+    from oracle import oracle as orc
+
+    n = A.size
+    P = n - m + 1
+    base = (m // 4 if excl < 0 else excl) + 1
+    Ap = np.append(A, 0.0)  # the last update of a diagonal reads one past the series
+    s, mu = orc.window_stats(A, m)
+    for e in entries:
+        e = int(e)
+        part = int(idx[e])
+        assert part >= 0, (e, part)
+        d = abs(e - part)
+        d0, d1 = max(base, d - 7), min(P - 1, d + 7) + 1  # diagonals [d0, d1)
+        tl = P - d0
+        qt = orc.seeds(A, m, d0, d1)
+        pc = np.full(tl, -np.inf)
+        ic = np.full(tl, -1, dtype=np.int64)
+        pr = np.full(tl, -np.inf)
+        ir = np.full(tl, -1, dtype=np.int64)
+        q = qt[d0:d1].copy()
+        orc.eval_block("scalar", pc, ic, pr, ir, q, d1 - d0, tl, Ap, Ap[d0:], m, s, mu, s[d0:], mu[d0:], d0, 0)
+        v = pc[e] if part > e else pr[e - d0]  # the entry as the column, or as the row, of its cells
+        assert 2.0 * m - 2.0 * v == prof[e], (e, part, prof[e], 2.0 * m - 2.0 * v)

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import thesiscode_amd as tc
-from conftest import assert_indices_match, cpu_threads, golden_cases
+from conftest import assert_indices_match, assert_values_equal_reference_walk, cpu_threads, golden_cases
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -201,6 +201,8 @@ def test_config2_full_size_properties():
     rows = np.linspace(0, P - 1, 64).astype(np.int64)
     for r in rows:
         assert abs(p[r] - orc.pair_dist2(A, m, r, i[r])) <= 1e-6 * max(1.0, p[r])
+    # bit for bit at 16 rows: the reference's arithmetic along the partners' diagonals
+    assert_values_equal_reference_walk(A, m, p, i, rows[::4])
     # full row brute force on a few rows: rho_j = sum_t (A[j+t]-mu_j) x_t / (m sd_j)
     mu, sd = orc.window_moments_ld(A, m)
     for r in rows[::16]:

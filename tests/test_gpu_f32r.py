@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import thesiscode_amd as tc
-from conftest import assert_indices_match
+from conftest import assert_indices_match, assert_values_equal_reference_walk
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
@@ -95,36 +95,12 @@ def config3_exact():
 
 def test_config3_exact_values_equal_reference_arithmetic(config3_exact):
     # At full size (n = 2^22, m = 1024) the exact path's value at sampled
-    # entries equals the reference's arithmetic bit for bit.  The oracle walks
-    # the 15 diagonals around the entry's partner from their first cells (the
-    # reference's seeds and recurrence over up to 4.2M steps, kernels.cpp:310-366)
-    # with the reference's separate-view adapter; the entry's cells on its
-    # partner's side include the winning lane's 8 candidates, so their maximum
-    # is the profile value.
+    # entries (10 random, the smallest and the largest) equals the reference's
+    # arithmetic over diagonals walked for up to 4.2M steps, bit for bit
     A, m, p64, i64 = config3_exact
-    n = A.size
-    P = n - m + 1
-    base = m // 4 + 1
-    Ap = np.append(A, 0.0)  # the last update of a diagonal reads one past the series
-    s, mu = orc.window_stats(A, m)
     rng = np.random.default_rng(12)
-    entries = list(rng.choice(P, 10, replace=False)) + [int(np.argmin(p64)), int(np.argmax(p64))]
-    for e in entries:
-        part = int(i64[e])
-        assert part >= 0
-        d = abs(e - part)
-        d0, d1 = max(base, d - 7), min(P - 1, d + 7) + 1  # diagonals [d0, d1)
-        tl = P - d0
-        qt = orc.seeds(A, m, d0, d1)
-        pc = np.full(tl, -np.inf)
-        ic = np.full(tl, -1, dtype=np.int64)
-        pr = np.full(tl, -np.inf)
-        ir = np.full(tl, -1, dtype=np.int64)
-        q = qt[d0:d1].copy()
-        orc.eval_block("scalar", pc, ic, pr, ir, q, d1 - d0, tl, Ap, Ap[d0:], m, s, mu, s[d0:], mu[d0:], d0, 0)
-        v = pc[e] if part > e else pr[e - d0]  # the entry as the column, or as the row, of its cells
-        print(f"entry {e} partner {part}: {p64[e]!r} vs {2.0 * m - 2.0 * v!r}", flush=True)
-        assert 2.0 * m - 2.0 * v == p64[e]
+    entries = list(rng.choice(p64.size, 10, replace=False)) + [int(np.argmin(p64)), int(np.argmax(p64))]
+    assert_values_equal_reference_walk(A, m, p64, i64, entries)
 
 
 # B = 0: the library's default re-seed interval (the longest automatic one, 8192)
