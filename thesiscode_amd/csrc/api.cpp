@@ -105,16 +105,18 @@ int geometry(int64_t n, int32_t m, const mp_opts* o, Geometry* g) {
     return MP_OK;
 }
 
-// Band width of the exact walker.  A work item is a pair of bands of 64*K
-// diagonals, so all items hold the same W*64K cells.  With I items on S SIMDs
-// and at most `occ` waves of the kernel per SIMD, s = min(occ, ceil(I/S)) waves
-// share a SIMD and the walk takes ceil(I / (S*s)) rounds of one item each:
-// time ~ rounds * 64K * c(K, s), c the measured cost per cell of one wave of
+// Band width of the exact walker.  A work item is one band of 64*K diagonals,
+// handed out longest first; the nb bands hold about as many cells as I =
+// ceil(nb/2) bands of the longest one's W columns.  With S SIMDs and at most
+// `occ` waves of the kernel per SIMD, s = min(occ, ceil(I/S)) waves share a
+// SIMD, the waves end together, and the walk takes max(1, I / (S*s)) times the
+// longest band: time ~ max(1, I/(S*s)) * W * 64K * c(K, s), W common to every
+// K and left out, c the measured cost per cell of one wave of
 // width K while s waves share its SIMD (MI355X, ns per cell): one wave alone
 // 2.19, 1.30, 0.75 (K = 1, 2, 4) and 0.38 for K = 8 with its LDS row ring;
 // s waves run at s * g(s) times the single-wave cost per wave, g = 1, 0.69,
 // 0.57, 0.49 for s = 1..4; K = 8 at two waves per SIMD (used when there are
-// more items than SIMDs, in its rotating form): 0.608 per wave.  Pick the K
+// more items than SIMDs, in its rotating form): 0.52 per wave.  Pick the K
 // that minimises it.  `form` (mp_opts.walk_form) picks the K = 8 form; 0
 // chooses the LDS ring at one wave per SIMD, the rotating form at two.
 struct WalkChoice {
@@ -137,12 +139,13 @@ WalkChoice pick_band_k_f64(const mp_ctx* c, int W, int requested, int form) {
     for (int i = 0; i < 4; ++i) {
         const int k = 1 << i;
         const int64_t items = items_of(k);
-        const bool pair = (k == 8 && items > S);
-        const int occ = (k == 8) ? (pair ? 2 : 1) : 4;
+        const bool two = (k == 8 && items > S);
+        const int occ = (k == 8) ? (two ? 2 : 1) : 4;
         const int64_t s = std::max<int64_t>(1, std::min<int64_t>(occ, (items + S - 1) / S));
-        const int64_t rounds = (items + S * s - 1) / (S * s);
-        const double per_wave = pair ? 0.608 : c1[i] * static_cast<double>(s) * g[s];
-        const double t = static_cast<double>(rounds) * 64.0 * k * per_wave;
+        const double rounds =
+            std::max(1.0, static_cast<double>(items) / static_cast<double>(S * s));
+        const double per_wave = two ? 0.52 : c1[i] * static_cast<double>(s) * g[s];
+        const double t = rounds * 64.0 * k * per_wave;
         if (i == 0 || t < best_t) {
             best_t = t;
             best = WalkChoice{k, form_of(k)};
@@ -334,9 +337,9 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
         a.writeback = 0;
         a.band_k = band_k;
         const int nb = mp::walk_f64_nbands(W, band_k);
-        const int items = (nb + 1) / 2;
+        const int waves = (nb + 1) / 2;  // at most one wave per two bands
         a.form = wc.form;
-        a.groups = std::max(1, std::min((items + 3) / 4, ((a.form == 2 || a.form == 3) ? 2 : 8) * c->cus));
+        a.groups = std::max(1, std::min((waves + 3) / 4, ((a.form == 2 || a.form == 3) ? 2 : 8) * c->cus));
         MP_TRY(mp::launch_walk_f64(a, st));
     }
     MP_HIP(hipEventRecord(c->ev[3], st));

@@ -448,7 +448,6 @@ __device__ __forceinline__ void walk_body(
     const int lane = threadIdx.x % kWave;
     const double NEG = -__builtin_inf();
     const double QNAN = __builtin_nan("");
-    const int nitems = (nbands + 1) / 2;
     const int lid0 = lane * K;
     double* const cv_out = &s_cv[wv][0][lane];
     double* const rv_out = &s_rv[wv][(lane == 0) ? 0 : F + lane];
@@ -536,287 +535,284 @@ __device__ __forceinline__ void walk_body(
     };
 
     for (;;) {
-        const int item = next_item(counter, lane);
-        if (item >= nitems) break;
-        // long/short pairing of bands (the reference pairs partitions r and 2p-1-r,
-        // ScrimpTrivialPar.cpp:216-224) keeps the work per item even.
-        for (int h = 0; h < 2; ++h) {
-            const int band = (h == 0) ? item : nbands - 1 - item;
-            if (h == 1 && band == item) break;
-            const int dbase = band * BW;
-            const int L = tl - dbase;
-            // Cells outside the triangle are masked by NaN: a row past the last
-            // one is a padding record with sInv = NaN, a diagonal past the block
-            // gets QT = NaN, and a NaN score never wins a comparison (nor does
-            // it in the reference).
-            // per row slot: rin = the row's maximum over the lanes above this
-            // one (exact), tag = the lane that holds it + 1 (0: none), lm = the
-            // maximum over this lane's cells of the row so far
-            double q[K], wa[K], wam[K], ws[K], wm[K], rin[K], lm[K];
-            int tag[K];
+        // One band per item, longest first (band 0 has the most columns):
+        // waves that finish early take the remaining, shorter bands, so all
+        // waves end within about one short band of each other.
+        const int band = next_item(counter, lane);
+        if (band >= nbands) break;
+        const int dbase = band * BW;
+        const int L = tl - dbase;
+        // Cells outside the triangle are masked by NaN: a row past the last
+        // one is a padding record with sInv = NaN, a diagonal past the block
+        // gets QT = NaN, and a NaN score never wins a comparison (nor does
+        // it in the reference).
+        // per row slot: rin = the row's maximum over the lanes above this
+        // one (exact), tag = the lane that holds it + 1 (0: none), lm = the
+        // maximum over this lane's cells of the row so far
+        double q[K], wa[K], wam[K], ws[K], wm[K], rin[K], lm[K];
+        int tag[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int d = dbase + lid0 + k;
+            q[k] = (d < blocklen) ? qt[d] : QNAN;
+            const WalkRec r = rowrec[d];
+            wa[k] = r.a;
+            wam[k] = r.am;
+            ws[k] = r.s;
+            wm[k] = r.mu;
+            rin[k] = NEG;
+            lm[k] = NEG;
+            tag[k] = 0;
+            if constexpr (RING) {  // rows 0 .. BW-1 of the band, slot = row - dbase
+                ring_put(lid0 + k, r);
+                if (lid0 + k < F) ring_put(RR + lid0 + k, r);
+            }
+        }
+        if constexpr (RING) {
+            // rows BW .. BW+F-1 of the band to the ring, rows BW+F .. BW+2F-1
+            // (the first window's new rows) to the staging array
+            const WalkRec r = rowrec[dbase + BW + (lane & (F - 1))];
+            const WalkRec r2 = rowrec[dbase + BW + F + (lane & (F - 1))];
+            if (lane < F) {
+                ring_put(BW + lane, r);
+                stage[lane] = r2;
+            }
+        }
+
+        // One column j = c0 + W for all K diagonals of every lane; W is the
+        // step's place in the flush window, so the row slot (W mod K) and
+        // the LDS offsets are constants.  TAIL: the window may end before
+        // F steps.  Column records are a linear scalar stream.
+        WalkRec cur_c, cur_f;  // the next step's column and feed records
+        int cur_nh = 0;        // the next step's fill word
+        int rbase = 0;         // ring byte offset of this window's first refill row
+        const WalkRec* fw = feed;  // this window's staged feed rows (FEEDLDS)
+        // A step's column partial and the row leaving lane 0 are written to
+        // LDS by the next step, after its wait for the scalar records: the
+        // wait (lgkmcnt, which also counts LDS writes) then never waits for
+        // writes issued just before it.
+        double pcv = NEG, prv = NEG;
+        int pri = 0;
+        auto put_partials = [&](int w) __attribute__((always_inline)) {
+            cv_out[w * (kWave + 1)] = pcv;
+            rv_out[w] = prv;
+            ri_out[w] = pri;
+        };
+        auto step = [&](int c0, auto w_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
+            constexpr int W = decltype(w_tag)::value;
+            constexpr int S = W % K;
+            constexpr bool TAIL = decltype(tail_tag)::value;
+            const int j = c0 + W;
+            if (TAIL && j >= L) return false;
+            // The column's record (a scalar load) and the row entering
+            // lane 63 (a padding record past the triangle; a scalar load,
+            // or an LDS read of the staged rows; the ring path reads rows
+            // from LDS instead) were loaded one step ago; the next step's are loaded now, so
+            // the loads overlap a step.  Steps are not interleaved by the
+            // scheduler: that keeps each step's use of its records (and so
+            // the wait for them) after the previous step's loads have had
+            // a whole step to land.
+            __builtin_amdgcn_sched_barrier(0);
+            const WalkRec c = cur_c, f = cur_f;
+            const int fnh = cur_nh;
+            // Scalar loads may return out of order, so a wait for this step's
+            // records waits for every scalar and LDS load in flight: take
+            // the wait here (the empty asm needs the values), before the
+            // next step's loads are issued (its memory clobber keeps them
+            // below).
+            if constexpr (RING || FEEDLDS) {
+                asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu) : "memory");
+            } else {
+                asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu), "s"(f.a), "s"(f.am),
+                             "s"(f.s), "s"(f.mu)
+                             : "memory");
+            }
+            cur_c = colrec[j + 1];
+            if constexpr (!RING && !FEEDLDS) cur_f = rowrec[j + 1 + dbase + BW];
+            // the last step's successor is the next window's first step,
+            // which reads these when the window starts
+            if constexpr (W + 1 < F) {
+                if constexpr (FEEDLDS) cur_f = fw[W + 1];
+                cur_nh = *neghi;
+            }
+            // the row this lane takes at the end of the step: row
+            // c0 + W + dbase + (lane+1)*K, which lane+1 finishes now; read
+            // before the step's LDS writes, so the read overlaps the step
+            WalkRec nr;
+            if constexpr (RING) {
+                // slot rbase + W, rbase a multiple of 8: a constant offset
+                const double2* p =
+                    reinterpret_cast<const double2*>(ringb + rbase + 32 * W + 16 * (W >> 3));
+                const double2 lo = p[0], hi = p[1];
+                nr.a = lo.x;
+                nr.am = lo.y;
+                nr.s = hi.x;
+                nr.mu = hi.y;
+            }
+            if constexpr (W > 0) put_partials(W - 1);
+            __builtin_amdgcn_sched_barrier(0);  // issue them before the step's work
+            const double ahj = c.a, ahjm = c.am, shj = c.s, mhj = c.mu;
+            double sc[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                const int d = dbase + lid0 + k;
-                q[k] = (d < blocklen) ? qt[d] : QNAN;
-                const WalkRec r = rowrec[d];
-                wa[k] = r.a;
-                wam[k] = r.am;
-                ws[k] = r.s;
-                wm[k] = r.mu;
-                rin[k] = NEG;
-                lm[k] = NEG;
-                tag[k] = 0;
-                if constexpr (RING) {  // rows 0 .. BW-1 of the band, slot = row - dbase
-                    ring_put(lid0 + k, r);
-                    if (lid0 + k < F) ring_put(RR + lid0 + k, r);
+                const int sl = (S + k) % K;
+                sc[k] = fma(q[k], ws[sl] * shj, -(wm[sl] * mhj));
+                q[k] = fma(wam[sl], ahjm, q[k]);
+                q[k] = fma(-wa[sl], ahj, q[k]);
+                if constexpr (WRITEBACK) {
+                    if (j == tl - 1 - (dbase + lid0 + k) && dbase + lid0 + k < blocklen)
+                        qt[dbase + lid0 + k] = q[k];
                 }
+                // the row in slot (S + K - 1) % K entered this lane at the end
+                // of the last step: its first cell here starts lm
+                lm[sl] = (k == K - 1) ? sc[k] : dmax(lm[sl], sc[k]);
             }
+            pcv = max_tree<K>(sc);
+            // the row in slot S leaves this lane: its maximum so far, and
+            // this lane as the argmax's when its own cells beat the lanes
+            // above (strictly: the row met those lanes first, at smaller
+            // columns); a NaN lm (only masked cells) changes nothing
+            prv = dmax(rin[S], lm[S]);
+            pri = (lm[S] > rin[S]) ? lane + 1 : tag[S];
+            rin[S] = shift_down_f64_lo0(prv, fnh);  // -inf into lane 63
+            tag[S] = shift_down_i32_zero(pri);
             if constexpr (RING) {
-                // rows BW .. BW+F-1 of the band to the ring, rows BW+F .. BW+2F-1
-                // (the first window's new rows) to the staging array
-                const WalkRec r = rowrec[dbase + BW + (lane & (F - 1))];
-                const WalkRec r2 = rowrec[dbase + BW + F + (lane & (F - 1))];
+                wa[S] = nr.a;
+                wam[S] = nr.am;
+                ws[S] = nr.s;
+                wm[S] = nr.mu;
+            } else {
+                // FEEDLDS: f is in vector registers already; else one
+                // 64-bit move per double brings it there
+                if constexpr (FEEDLDS) {
+                    wa[S] = shift_down_f64(wa[S], f.a);
+                    wam[S] = shift_down_f64(wam[S], f.am);
+                    ws[S] = shift_down_f64(ws[S], f.s);
+                    wm[S] = shift_down_f64(wm[S], f.mu);
+                } else {
+                    wa[S] = shift_down_f64(wa[S], vmov64(f.a));
+                    wam[S] = shift_down_f64(wam[S], vmov64(f.am));
+                    ws[S] = shift_down_f64(ws[S], vmov64(f.s));
+                    wm[S] = shift_down_f64(wm[S], vmov64(f.mu));
+                }
+            }
+            return true;
+        };
+        auto window = [&](int c0, auto tail_tag) __attribute__((always_inline)) {
+            if constexpr (RING) {
+                // The next window's new rows, band rows c0 + F + BW + x, go
+                // from the staging array to slots (c0 + F + BW + x) mod RR,
+                // which held rows c0 - F + x: finished in the last window.
+                __builtin_amdgcn_wave_barrier();
                 if (lane < F) {
-                    ring_put(BW + lane, r);
-                    stage[lane] = r2;
+                    const WalkRec t = stage[lane];
+                    const int slot = (c0 + F + BW + lane) % RR;
+                    ring_put(slot, t);
+                    if (slot < F) ring_put(RR + slot, t);
                 }
-            }
-
-            // One column j = c0 + W for all K diagonals of every lane; W is the
-            // step's place in the flush window, so the row slot (W mod K) and
-            // the LDS offsets are constants.  TAIL: the window may end before
-            // F steps.  Column records are a linear scalar stream.
-            WalkRec cur_c, cur_f;  // the next step's column and feed records
-            int cur_nh = 0;        // the next step's fill word
-            int rbase = 0;         // ring byte offset of this window's first refill row
-            const WalkRec* fw = feed;  // this window's staged feed rows (FEEDLDS)
-            // A step's column partial and the row leaving lane 0 are written to
-            // LDS by the next step, after its wait for the scalar records: the
-            // wait (lgkmcnt, which also counts LDS writes) then never waits for
-            // writes issued just before it.
-            double pcv = NEG, prv = NEG;
-            int pri = 0;
-            auto put_partials = [&](int w) __attribute__((always_inline)) {
-                cv_out[w * (kWave + 1)] = pcv;
-                rv_out[w] = prv;
-                ri_out[w] = pri;
-            };
-            auto step = [&](int c0, auto w_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
-                constexpr int W = decltype(w_tag)::value;
-                constexpr int S = W % K;
-                constexpr bool TAIL = decltype(tail_tag)::value;
-                const int j = c0 + W;
-                if (TAIL && j >= L) return false;
-                // The column's record (a scalar load) and the row entering
-                // lane 63 (a padding record past the triangle; a scalar load,
-                // or an LDS read of the staged rows; the ring path reads rows
-                // from LDS instead) were loaded one step ago; the next step's are loaded now, so
-                // the loads overlap a step.  Steps are not interleaved by the
-                // scheduler: that keeps each step's use of its records (and so
-                // the wait for them) after the previous step's loads have had
-                // a whole step to land.
-                __builtin_amdgcn_sched_barrier(0);
-                const WalkRec c = cur_c, f = cur_f;
-                const int fnh = cur_nh;
-                // Scalar loads may return out of order, so a wait for this step's
-                // records waits for every scalar and LDS load in flight: take
-                // the wait here (the empty asm needs the values), before the
-                // next step's loads are issued (its memory clobber keeps them
-                // below).
-                if constexpr (RING || FEEDLDS) {
-                    asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu) : "memory");
-                } else {
-                    asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu), "s"(f.a), "s"(f.am),
-                                 "s"(f.s), "s"(f.mu)
-                                 : "memory");
-                }
-                cur_c = colrec[j + 1];
-                if constexpr (!RING && !FEEDLDS) cur_f = rowrec[j + 1 + dbase + BW];
-                // the last step's successor is the next window's first step,
-                // which reads these when the window starts
-                if constexpr (W + 1 < F) {
-                    if constexpr (FEEDLDS) cur_f = fw[W + 1];
-                    cur_nh = *neghi;
-                }
-                // the row this lane takes at the end of the step: row
-                // c0 + W + dbase + (lane+1)*K, which lane+1 finishes now; read
-                // before the step's LDS writes, so the read overlaps the step
-                WalkRec nr;
-                if constexpr (RING) {
-                    // slot rbase + W, rbase a multiple of 8: a constant offset
-                    const double2* p =
-                        reinterpret_cast<const double2*>(ringb + rbase + 32 * W + 16 * (W >> 3));
-                    const double2 lo = p[0], hi = p[1];
-                    nr.a = lo.x;
-                    nr.am = lo.y;
-                    nr.s = hi.x;
-                    nr.mu = hi.y;
-                }
-                if constexpr (W > 0) put_partials(W - 1);
-                __builtin_amdgcn_sched_barrier(0);  // issue them before the step's work
-                const double ahj = c.a, ahjm = c.am, shj = c.s, mhj = c.mu;
-                double sc[K];
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const int sl = (S + k) % K;
-                    sc[k] = fma(q[k], ws[sl] * shj, -(wm[sl] * mhj));
-                    q[k] = fma(wam[sl], ahjm, q[k]);
-                    q[k] = fma(-wa[sl], ahj, q[k]);
-                    if constexpr (WRITEBACK) {
-                        if (j == tl - 1 - (dbase + lid0 + k) && dbase + lid0 + k < blocklen)
-                            qt[dbase + lid0 + k] = q[k];
-                    }
-                    // the row in slot (S + K - 1) % K entered this lane at the end
-                    // of the last step: its first cell here starts lm
-                    lm[sl] = (k == K - 1) ? sc[k] : dmax(lm[sl], sc[k]);
-                }
-                pcv = max_tree<K>(sc);
-                // the row in slot S leaves this lane: its maximum so far, and
-                // this lane as the argmax's when its own cells beat the lanes
-                // above (strictly: the row met those lanes first, at smaller
-                // columns); a NaN lm (only masked cells) changes nothing
-                prv = dmax(rin[S], lm[S]);
-                pri = (lm[S] > rin[S]) ? lane + 1 : tag[S];
-                rin[S] = shift_down_f64_lo0(prv, fnh);  // -inf into lane 63
-                tag[S] = shift_down_i32_zero(pri);
-                if constexpr (RING) {
-                    wa[S] = nr.a;
-                    wam[S] = nr.am;
-                    ws[S] = nr.s;
-                    wm[S] = nr.mu;
-                } else {
-                    // FEEDLDS: f is in vector registers already; else one
-                    // 64-bit move per double brings it there
-                    if constexpr (FEEDLDS) {
-                        wa[S] = shift_down_f64(wa[S], f.a);
-                        wam[S] = shift_down_f64(wam[S], f.am);
-                        ws[S] = shift_down_f64(ws[S], f.s);
-                        wm[S] = shift_down_f64(wm[S], f.mu);
-                    } else {
-                        wa[S] = shift_down_f64(wa[S], vmov64(f.a));
-                        wam[S] = shift_down_f64(wam[S], vmov64(f.am));
-                        ws[S] = shift_down_f64(ws[S], vmov64(f.s));
-                        wm[S] = shift_down_f64(wm[S], vmov64(f.mu));
-                    }
-                }
-                return true;
-            };
-            auto window = [&](int c0, auto tail_tag) __attribute__((always_inline)) {
-                if constexpr (RING) {
-                    // The next window's new rows, band rows c0 + F + BW + x, go
-                    // from the staging array to slots (c0 + F + BW + x) mod RR,
-                    // which held rows c0 - F + x: finished in the last window.
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane < F) {
-                        const WalkRec t = stage[lane];
-                        const int slot = (c0 + F + BW + lane) % RR;
-                        ring_put(slot, t);
-                        if (slot < F) ring_put(RR + slot, t);
-                    }
-                    // then the staging array takes band rows c0 + 2F + BW + x
-                    // (padding records past the triangle), 16 bytes per lane;
-                    // the staging read above has completed before the wave goes on
-                    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-                    if (lane < 2 * F) {
-                        const char* src = reinterpret_cast<const char*>(
-                                              rowrec + (dbase + c0 + 2 * F + BW + (lane >> 1))) +
-                                          16 * (lane & 1);
-                        __builtin_amdgcn_global_load_lds(
-                            (const __attribute__((address_space(1))) void*)(src),
-                            (__attribute__((address_space(3))) void*)(stage), 16, 0, 0);
-                        // the current results of this window's columns and rows,
-                        // for the flush's read filter (4 bytes per lane, clamped)
-                        const int jj = min(c0 + (lane >> 1), L - 1);
-                        const char* cs = reinterpret_cast<const char*>(colval + jj) + 4 * (lane & 1);
-                        const char* rs = reinterpret_cast<const char*>(rowval + jj + dbase) + 4 * (lane & 1);
-                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(cs),
-                                                         (__attribute__((address_space(3))) void*)(filt), 4, 0, 0);
-                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rs),
-                                                         (__attribute__((address_space(3))) void*)(filt + F), 4,
-                                                         0, 0);
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    rbase = ring_off((c0 + lid0 + K) % RR);  // a byte offset
-                    cur_nh = *neghi;
-                } else if constexpr (!FEEDLDS) {
-                    cur_nh = *neghi;
-                } else {
-                    // The next window's feed rows (band rows c0 + F + BW + x,
-                    // padding records past the triangle) go to the other
-                    // buffer, 16 bytes per lane; this window's landed before
-                    // the last flush.
-                    __builtin_amdgcn_wave_barrier();
-                    const int nb = (c0 / F) & 1;
-                    if (lane < 2 * F) {
-                        const char* src = reinterpret_cast<const char*>(
-                                              rowrec + (dbase + c0 + F + BW + (lane >> 1))) +
-                                          16 * (lane & 1);
-                        __builtin_amdgcn_global_load_lds(
-                            (const __attribute__((address_space(1))) void*)(src),
-                            (__attribute__((address_space(3))) void*)(feed + (nb ^ 1) * F), 16, 0, 0);
-                    }
-                    fw = feed + nb * F;
-                    cur_f = fw[0];
-                    cur_nh = *neghi;
-                    __builtin_amdgcn_wave_barrier();
-                }
-                bool go = true;
-                [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
-                    ((go = go && step(c0, std::integral_constant<int, W>{}, tail_tag)), ...);
-                }(std::make_integer_sequence<int, F>{});
-                // the window's last step writes no successor's partials
-                if constexpr (decltype(tail_tag)::value) {
-                    put_partials(min(F, L - c0) - 1);
-                } else {
-                    put_partials(F - 1);
-                }
-            };
-            // results of column c0 + (lane mod F) and of its row, clamped in bounds
-            auto current = [&](int c0, uint64_t& pc, uint64_t& pr) __attribute__((always_inline)) {
-                const int jj = min(c0 + (lane & (F - 1)), L - 1);
-                pc = colval[jj];
-                pr = rowval[jj + dbase];
-            };
-            int c0 = 0;
-            uint64_t pc = 0, pr = 0;
-            if constexpr (!RING) current(0, pc, pr);
-            if constexpr (FEEDLDS) {
-                // the first window's feed rows (band rows BW + x) to buffer 0
-                __builtin_amdgcn_wave_barrier();
+                // then the staging array takes band rows c0 + 2F + BW + x
+                // (padding records past the triangle), 16 bytes per lane;
+                // the staging read above has completed before the wave goes on
+                __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
                 if (lane < 2 * F) {
-                    const char* src =
-                        reinterpret_cast<const char*>(rowrec + (dbase + BW + (lane >> 1))) + 16 * (lane & 1);
-                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
-                                                     (__attribute__((address_space(3))) void*)(feed), 16, 0, 0);
+                    const char* src = reinterpret_cast<const char*>(
+                                          rowrec + (dbase + c0 + 2 * F + BW + (lane >> 1))) +
+                                      16 * (lane & 1);
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void*)(src),
+                        (__attribute__((address_space(3))) void*)(stage), 16, 0, 0);
+                    // the current results of this window's columns and rows,
+                    // for the flush's read filter (4 bytes per lane, clamped)
+                    const int jj = min(c0 + (lane >> 1), L - 1);
+                    const char* cs = reinterpret_cast<const char*>(colval + jj) + 4 * (lane & 1);
+                    const char* rs = reinterpret_cast<const char*>(rowval + jj + dbase) + 4 * (lane & 1);
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(cs),
+                                                     (__attribute__((address_space(3))) void*)(filt), 4, 0, 0);
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(rs),
+                                                     (__attribute__((address_space(3))) void*)(filt + F), 4,
+                                                     0, 0);
                 }
                 __builtin_amdgcn_wave_barrier();
-            }
-            cur_c = colrec[0];
-            if constexpr (!RING && !FEEDLDS) cur_f = rowrec[dbase + BW];
-            // let the band's set-up loads land here, so the windows never wait on
-            // vector memory (whose counter would also cover the flush's atomics)
-            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-            // whole flush windows: no control flow inside a window
-            for (; c0 + F <= L; c0 += F) {
-                window(c0, std::false_type{});
-                // the direct-to-LDS loads of this window (the next rows, and
-                // with the ring the filter values) must have landed: the
-                // compiler does not order LDS reads after them
-                if constexpr (RING || FEEDLDS) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                if constexpr (RING) {
-                    flush(c0, F, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
-                } else {
-                    flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
-                        current(c0 + F, a, b);
-                    });
+                rbase = ring_off((c0 + lid0 + K) % RR);  // a byte offset
+                cur_nh = *neghi;
+            } else if constexpr (!FEEDLDS) {
+                cur_nh = *neghi;
+            } else {
+                // The next window's feed rows (band rows c0 + F + BW + x,
+                // padding records past the triangle) go to the other
+                // buffer, 16 bytes per lane; this window's landed before
+                // the last flush.
+                __builtin_amdgcn_wave_barrier();
+                const int nb = (c0 / F) & 1;
+                if (lane < 2 * F) {
+                    const char* src = reinterpret_cast<const char*>(
+                                          rowrec + (dbase + c0 + F + BW + (lane >> 1))) +
+                                      16 * (lane & 1);
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void*)(src),
+                        (__attribute__((address_space(3))) void*)(feed + (nb ^ 1) * F), 16, 0, 0);
                 }
+                fw = feed + nb * F;
+                cur_f = fw[0];
+                cur_nh = *neghi;
+                __builtin_amdgcn_wave_barrier();
             }
-            // the last partial window
-            if (c0 < L) {
-                window(c0, std::true_type{});
-                if constexpr (RING || FEEDLDS) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
+            bool go = true;
+            [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
+                ((go = go && step(c0, std::integral_constant<int, W>{}, tail_tag)), ...);
+            }(std::make_integer_sequence<int, F>{});
+            // the window's last step writes no successor's partials
+            if constexpr (decltype(tail_tag)::value) {
+                put_partials(min(F, L - c0) - 1);
+            } else {
+                put_partials(F - 1);
             }
+        };
+        // results of column c0 + (lane mod F) and of its row, clamped in bounds
+        auto current = [&](int c0, uint64_t& pc, uint64_t& pr) __attribute__((always_inline)) {
+            const int jj = min(c0 + (lane & (F - 1)), L - 1);
+            pc = colval[jj];
+            pr = rowval[jj + dbase];
+        };
+        int c0 = 0;
+        uint64_t pc = 0, pr = 0;
+        if constexpr (!RING) current(0, pc, pr);
+        if constexpr (FEEDLDS) {
+            // the first window's feed rows (band rows BW + x) to buffer 0
+            __builtin_amdgcn_wave_barrier();
+            if (lane < 2 * F) {
+                const char* src =
+                    reinterpret_cast<const char*>(rowrec + (dbase + BW + (lane >> 1))) + 16 * (lane & 1);
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
+                                                 (__attribute__((address_space(3))) void*)(feed), 16, 0, 0);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        cur_c = colrec[0];
+        if constexpr (!RING && !FEEDLDS) cur_f = rowrec[dbase + BW];
+        // let the band's set-up loads land here, so the windows never wait on
+        // vector memory (whose counter would also cover the flush's atomics)
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        // whole flush windows: no control flow inside a window
+        for (; c0 + F <= L; c0 += F) {
+            window(c0, std::false_type{});
+            // the direct-to-LDS loads of this window (the next rows, and
+            // with the ring the filter values) must have landed: the
+            // compiler does not order LDS reads after them
+            if constexpr (RING || FEEDLDS) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            if constexpr (RING) {
+                flush(c0, F, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
+            } else {
+                flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
+                    current(c0 + F, a, b);
+                });
+            }
+        }
+        // the last partial window
+        if (c0 < L) {
+            window(c0, std::true_type{});
+            if constexpr (RING || FEEDLDS) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
         }
     }
 }
@@ -894,7 +890,6 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
     const int lane = threadIdx.x % kWave;
     const double NEG = -__builtin_inf();
     const double QNAN = __builtin_nan("");
-    const int nitems = (nbands + 1) / 2;
     const int lid0 = lane * K;
     WalkRec* const feed = &s_feed[wv][0];
 
@@ -956,160 +951,159 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
     };
 
     for (;;) {
-        const int item = next_item(counter, lane);
-        if (item >= nitems) break;
-        for (int h = 0; h < 2; ++h) {
-            const int band = (h == 0) ? item : nbands - 1 - item;
-            if (h == 1 && band == item) break;
-            const int dbase = band * BW;
-            const int L = tl - dbase;
-            double qr[K], wa[K], wam[K], ws[K], wm[K], lm[K], run[K];
-            int tag[K];
+        // One band per item, longest first (band 0 has the most columns):
+        // waves that finish early take the remaining, shorter bands, so all
+        // waves end within about one short band of each other.
+        const int band = next_item(counter, lane);
+        if (band >= nbands) break;
+        const int dbase = band * BW;
+        const int L = tl - dbase;
+        double qr[K], wa[K], wam[K], ws[K], wm[K], lm[K], run[K];
+        int tag[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int d = dbase + lid0 + k;  // at column 0 position e holds diagonal and row e
-                qr[k] = (d < blocklen) ? qt[d] : QNAN;
-                const WalkRec r = rowrec[d];
-                wa[k] = r.a;
-                wam[k] = r.am;
-                ws[k] = r.s;
-                wm[k] = r.mu;
-                lm[k] = NEG;
-                run[k] = NEG;
-                tag[k] = -1;
+        for (int k = 0; k < K; ++k) {
+            const int d = dbase + lid0 + k;  // at column 0 position e holds diagonal and row e
+            qr[k] = (d < blocklen) ? qt[d] : QNAN;
+            const WalkRec r = rowrec[d];
+            wa[k] = r.a;
+            wam[k] = r.am;
+            ws[k] = r.s;
+            wm[k] = r.mu;
+            lm[k] = NEG;
+            run[k] = NEG;
+            tag[k] = -1;
+        }
+        WalkRec cur_c;             // the next step's column record
+        const WalkRec* fw = feed;  // this window's entering rows
+        int wl0 = 0;               // the wrap lane of the window's first column
+        // this lane's rotated index, per half window (its wrap lane)
+        int rot = 0;
+        auto half = [&](int wl) __attribute__((always_inline)) { rot = (lane - wl) & (kWave - 1); };
+        // A step's column pair is written by the next step, after its wait
+        // for the scalar records (as in walk_body).
+        double pav = NEG, pbv = NEG;
+        auto put_pair = [&](int w, int r) __attribute__((always_inline)) {
+            s_cv[wv][w][r] = make_double2(pav, pbv);
+        };
+        auto step = [&](int c0, auto w_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
+            constexpr int W = decltype(w_tag)::value;
+            constexpr int S = W % K;             // the wrap slot: its row leaves after this step
+            constexpr int S0 = (W + K - 1) % K;  // the slot whose block starts at this step
+            constexpr bool TAIL = decltype(tail_tag)::value;
+            const int j = c0 + W;
+            if (TAIL && j >= L) return false;
+            __builtin_amdgcn_sched_barrier(0);
+            // Scalar loads may return out of order, so the wait for this
+            // step's record (forced by the empty asm, which needs the
+            // values) covers every scalar and LDS operation in flight; the
+            // next step's record is loaded after it.
+            const WalkRec c = cur_c;
+            asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu) : "memory");
+            cur_c = colrec[j + 1];
+            if constexpr (W > 0) {
+                const int rprev = rot;  // the last step's wrap lane
+                if constexpr (S == 0) half(wl0 + W / K);
+                put_pair(W - 1, rprev);
             }
-            WalkRec cur_c;             // the next step's column record
-            const WalkRec* fw = feed;  // this window's entering rows
-            int wl0 = 0;               // the wrap lane of the window's first column
-            // this lane's rotated index, per half window (its wrap lane)
-            int rot = 0;
-            auto half = [&](int wl) __attribute__((always_inline)) { rot = (lane - wl) & (kWave - 1); };
-            // A step's column pair is written by the next step, after its wait
-            // for the scalar records (as in walk_body).
-            double pav = NEG, pbv = NEG;
-            auto put_pair = [&](int w, int r) __attribute__((always_inline)) {
-                s_cv[wv][w][r] = make_double2(pav, pbv);
+            __builtin_amdgcn_sched_barrier(0);
+            const double ahj = c.a, ahjm = c.am, shj = c.s, mhj = c.mu;
+            double sc[K];
+            auto cell = [&](auto s_tag) __attribute__((always_inline)) {
+                constexpr int s = decltype(s_tag)::value;
+                constexpr int t = (s - S + K) % K;  // the register of slot s's diagonal
+                sc[s] = fma(qr[t], ws[s] * shj, -(wm[s] * mhj));
+                qr[t] = fma(wam[s], ahjm, qr[t]);
+                qr[t] = fma(-wa[s], ahj, qr[t]);
+                lm[s] = (s == S0) ? sc[s] : dmax(lm[s], sc[s]);
             };
-            auto step = [&](int c0, auto w_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
-                constexpr int W = decltype(w_tag)::value;
-                constexpr int S = W % K;             // the wrap slot: its row leaves after this step
-                constexpr int S0 = (W + K - 1) % K;  // the slot whose block starts at this step
-                constexpr bool TAIL = decltype(tail_tag)::value;
-                const int j = c0 + W;
-                if (TAIL && j >= L) return false;
-                __builtin_amdgcn_sched_barrier(0);
-                // Scalar loads may return out of order, so the wait for this
-                // step's record (forced by the empty asm, which needs the
-                // values) covers every scalar and LDS operation in flight; the
-                // next step's record is loaded after it.
-                const WalkRec c = cur_c;
-                asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu) : "memory");
-                cur_c = colrec[j + 1];
-                if constexpr (W > 0) {
-                    const int rprev = rot;  // the last step's wrap lane
-                    if constexpr (S == 0) half(wl0 + W / K);
-                    put_pair(W - 1, rprev);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-                const double ahj = c.a, ahjm = c.am, shj = c.s, mhj = c.mu;
-                double sc[K];
-                auto cell = [&](auto s_tag) __attribute__((always_inline)) {
-                    constexpr int s = decltype(s_tag)::value;
-                    constexpr int t = (s - S + K) % K;  // the register of slot s's diagonal
-                    sc[s] = fma(qr[t], ws[s] * shj, -(wm[s] * mhj));
-                    qr[t] = fma(wam[s], ahjm, qr[t]);
-                    qr[t] = fma(-wa[s], ahj, qr[t]);
-                    lm[s] = (s == S0) ? sc[s] : dmax(lm[s], sc[s]);
-                };
-                // the leaving row first, so that its successor's record loads
-                // while the other slots compute
-                cell(std::integral_constant<int, S>{});
-                // The QT updates read the rows: the compiler would sink them to
-                // their next use, the next step, past the masked region that
-                // replaces a row, and copy that row's registers to keep them.
-                // Empty asm statements that take the updated values keep them
-                // here (the leaving slot's) and in the step (the others).
-                asm volatile("" ::"v"(qr[0]));
-                tag[S] = !(lm[S] <= run[S]) ? j : tag[S];
-                run[S] = dmax(run[S], lm[S]);
-                if (__builtin_expect(rot == 0, 1)) {
-                    s_rr[wv][W].v = run[S];
-                    s_rr[wv][W].tag = tag[S];
-                    run[S] = __hiloint2double(static_cast<int>(0xFFF80000u), __double2loint(run[S]));
-                    const double2* p = reinterpret_cast<const double2*>(fw + W);
-                    const double2 lo = p[0], hi = p[1];
-                    wa[S] = lo.x;
-                    wam[S] = lo.y;
-                    ws[S] = hi.x;
-                    wm[S] = hi.y;
-                }
-                [&]<int... s>(std::integer_sequence<int, s...>) __attribute__((always_inline)) {
-                    ((s != S ? cell(std::integral_constant<int, s>{}) : void()), ...);
-                }(std::make_integer_sequence<int, K>{});
-                // slot K-1's diagonal moves to the next lane's slot 0
-                constexpr int tl7 = (K - 1 - S + K) % K;
-                qr[tl7] = rotate_up_f64(qr[tl7]);
+            // the leaving row first, so that its successor's record loads
+            // while the other slots compute
+            cell(std::integral_constant<int, S>{});
+            // The QT updates read the rows: the compiler would sink them to
+            // their next use, the next step, past the masked region that
+            // replaces a row, and copy that row's registers to keep them.
+            // Empty asm statements that take the updated values keep them
+            // here (the leaving slot's) and in the step (the others).
+            asm volatile("" ::"v"(qr[0]));
+            tag[S] = !(lm[S] <= run[S]) ? j : tag[S];
+            run[S] = dmax(run[S], lm[S]);
+            if (__builtin_expect(rot == 0, 1)) {
+                s_rr[wv][W].v = run[S];
+                s_rr[wv][W].tag = tag[S];
+                run[S] = __hiloint2double(static_cast<int>(0xFFF80000u), __double2loint(run[S]));
+                const double2* p = reinterpret_cast<const double2*>(fw + W);
+                const double2 lo = p[0], hi = p[1];
+                wa[S] = lo.x;
+                wam[S] = lo.y;
+                ws[S] = hi.x;
+                wm[S] = hi.y;
+            }
+            [&]<int... s>(std::integer_sequence<int, s...>) __attribute__((always_inline)) {
+                ((s != S ? cell(std::integral_constant<int, s>{}) : void()), ...);
+            }(std::make_integer_sequence<int, K>{});
+            // slot K-1's diagonal moves to the next lane's slot 0
+            constexpr int tl7 = (K - 1 - S + K) % K;
+            qr[tl7] = rotate_up_f64(qr[tl7]);
 #pragma unroll
-                for (int t = 1; t < K; ++t) asm volatile("" ::"v"(qr[t]));
-                pav = max_tree<K - S, S>(sc);
-                pbv = (S == 0) ? NEG : max_tree<(S > 0 ? S : 1), 0>(sc);
-                return true;
-            };
-            // rows dbase + c0 + BW + x, x < F (padding records past the
-            // triangle), to buffer nb: lane l < 2F loads half l % 2 of row l / 2
-            auto stage = [&](int c0, int nb) __attribute__((always_inline)) {
-                if (lane < 2 * F) {
-                    const char* src =
-                        reinterpret_cast<const char*>(rowrec + (dbase + c0 + BW + (lane >> 1))) + 16 * (lane & 1);
-                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
-                                                     (__attribute__((address_space(3))) void*)(feed + nb * F), 16,
-                                                     0, 0);
-                }
-            };
-            auto window = [&](int c0, auto tail_tag) __attribute__((always_inline)) {
-                // the next window's entering rows (band rows c0 + F + BW + x) to
-                // the other buffer, 16 bytes per lane; this window's landed
-                // before the last flush
-                __builtin_amdgcn_wave_barrier();
-                const int nb = (c0 / F) & 1;
-                stage(c0 + F, nb ^ 1);
-                fw = feed + nb * F;
-                wl0 = (c0 % BW) / K;
-                half(wl0);
-                __builtin_amdgcn_wave_barrier();
-                bool go = true;
-                [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
-                    ((go = go && step(c0, std::integral_constant<int, W>{}, tail_tag)), ...);
-                }(std::make_integer_sequence<int, F>{});
-                const int wlast = decltype(tail_tag)::value ? min(F, L - c0) - 1 : F - 1;
-                put_pair(wlast, rot);
-            };
-            auto current = [&](int c0, uint64_t& pc, uint64_t& pr) __attribute__((always_inline)) {
-                const int jj = min(c0 + (lane & (F - 1)), L - 1);
-                pc = colval[jj];
-                pr = rowval[jj + dbase];
-            };
-            int c0 = 0;
-            uint64_t pc = 0, pr = 0;
-            current(0, pc, pr);
-            // the first window's entering rows (band rows BW + x) to buffer 0
-            __builtin_amdgcn_wave_barrier();
-            stage(0, 0);
-            __builtin_amdgcn_wave_barrier();
-            cur_c = colrec[0];
-            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-            for (; c0 + F <= L; c0 += F) {
-                window(c0, std::false_type{});
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
-                    current(c0 + F, a, b);
-                });
+            for (int t = 1; t < K; ++t) asm volatile("" ::"v"(qr[t]));
+            pav = max_tree<K - S, S>(sc);
+            pbv = (S == 0) ? NEG : max_tree<(S > 0 ? S : 1), 0>(sc);
+            return true;
+        };
+        // rows dbase + c0 + BW + x, x < F (padding records past the
+        // triangle), to buffer nb: lane l < 2F loads half l % 2 of row l / 2
+        auto stage = [&](int c0, int nb) __attribute__((always_inline)) {
+            if (lane < 2 * F) {
+                const char* src =
+                    reinterpret_cast<const char*>(rowrec + (dbase + c0 + BW + (lane >> 1))) + 16 * (lane & 1);
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
+                                                 (__attribute__((address_space(3))) void*)(feed + nb * F), 16,
+                                                 0, 0);
             }
-            if (c0 < L) {
-                window(c0, std::true_type{});
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
-            }
+        };
+        auto window = [&](int c0, auto tail_tag) __attribute__((always_inline)) {
+            // the next window's entering rows (band rows c0 + F + BW + x) to
+            // the other buffer, 16 bytes per lane; this window's landed
+            // before the last flush
+            __builtin_amdgcn_wave_barrier();
+            const int nb = (c0 / F) & 1;
+            stage(c0 + F, nb ^ 1);
+            fw = feed + nb * F;
+            wl0 = (c0 % BW) / K;
+            half(wl0);
+            __builtin_amdgcn_wave_barrier();
+            bool go = true;
+            [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
+                ((go = go && step(c0, std::integral_constant<int, W>{}, tail_tag)), ...);
+            }(std::make_integer_sequence<int, F>{});
+            const int wlast = decltype(tail_tag)::value ? min(F, L - c0) - 1 : F - 1;
+            put_pair(wlast, rot);
+        };
+        auto current = [&](int c0, uint64_t& pc, uint64_t& pr) __attribute__((always_inline)) {
+            const int jj = min(c0 + (lane & (F - 1)), L - 1);
+            pc = colval[jj];
+            pr = rowval[jj + dbase];
+        };
+        int c0 = 0;
+        uint64_t pc = 0, pr = 0;
+        current(0, pc, pr);
+        // the first window's entering rows (band rows BW + x) to buffer 0
+        __builtin_amdgcn_wave_barrier();
+        stage(0, 0);
+        __builtin_amdgcn_wave_barrier();
+        cur_c = colrec[0];
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        for (; c0 + F <= L; c0 += F) {
+            window(c0, std::false_type{});
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
+                current(c0 + F, a, b);
+            });
+        }
+        if (c0 < L) {
+            window(c0, std::true_type{});
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
         }
     }
 }
