@@ -77,6 +77,39 @@ def test_band_width_8_walker_forms_bit_exact(walk_form, n, m, seed):
     assert_indices_match(A, m, p_ref, i_ref, i, abs_tol=_tie_tol(m), min_frac=0.999)
 
 
+# 15 N(0,1) values (numpy default_rng(5), the 677th of a search for them): with
+# m = 8, six of the eight profile entries have their best score in (-1, 0)
+NEG_SCORE_SERIES = [0.4970085297607596, -0.03761213355239576, -1.4097385957119746, 0.2362400362428747,
+                    -0.3585197480443938, 0.10868428288495499, 1.9872942438908143, -0.5601415338663673,
+                    0.4162561732853156, -0.3859331462233166, 0.1400420355261085, -0.9092892407234737,
+                    -0.13049969804198341, 0.013218314525724945, 1.8224528368681574]
+
+
+def test_rotating_form_results_that_stay_negative():
+    # The rotating walker tests each step's cells against the column's current
+    # result and lets every cell through while that result is not positive.
+    # (a) scores in (-1, 0): results that stay negative to the end;
+    # (b) every correlation below -1/m (a sine, partners a quarter to three
+    #     quarters of a period away): no cell beats the reference's initial
+    #     score -1, and every entry keeps d^2 = 2m + 2 with index -1.
+    A = np.array(NEG_SCORE_SERIES)
+    m = 8
+    p_ref, i_ref = _oracle(A, m)
+    assert int(((p_ref > 2 * m) & (p_ref < 2 * m + 2)).sum()) == 6
+    p, i, tm = tc.selfjoin_f64(A, m, band_k=8, walk_form=3)
+    assert tm["walk_form"] == 3
+    assert np.array_equal(p, p_ref)
+    assert np.array_equal(i, i_ref)
+    m, ez = 64, 16
+    n = 47 + m - 1
+    t = np.arange(n, dtype=np.float64)
+    A = np.sin(2 * np.pi * t / (4 * ez)) + 1e-3 * tc.random_walk(77, n)
+    p_ref, i_ref = _oracle(A, m)
+    assert np.all(p_ref == 2 * m + 2) and np.all(i_ref == -1)
+    p, i, _ = tc.selfjoin_f64(A, m, band_k=8, walk_form=3)
+    assert np.array_equal(p, p_ref) and np.array_equal(i, i_ref)
+
+
 def test_walk_form_out_of_range_is_refused():
     with pytest.raises(tc.ScrimpError):
         tc.selfjoin_f64(tc.random_walk(1, 2000), 32, band_k=8, walk_form=4)

@@ -116,7 +116,7 @@ int geometry(int64_t n, int32_t m, const mp_opts* o, Geometry* g) {
 // 2.19, 1.30, 0.75 (K = 1, 2, 4) and 0.38 for K = 8 with its LDS row ring;
 // s waves run at s * g(s) times the single-wave cost per wave, g = 1, 0.69,
 // 0.57, 0.49 for s = 1..4; K = 8 at two waves per SIMD (used when there are
-// more items than SIMDs, in its rotating form): 0.52 per wave.  Pick the K
+// more items than SIMDs, in its rotating form): 0.49 per wave.  Pick the K
 // that minimises it.  `form` (mp_opts.walk_form) picks the K = 8 form; 0
 // chooses the LDS ring at one wave per SIMD, the rotating form at two.
 struct WalkChoice {
@@ -144,7 +144,7 @@ WalkChoice pick_band_k_f64(const mp_ctx* c, int W, int requested, int form) {
         const int64_t s = std::max<int64_t>(1, std::min<int64_t>(occ, (items + S - 1) / S));
         const double rounds =
             std::max(1.0, static_cast<double>(items) / static_cast<double>(S * s));
-        const double per_wave = two ? 0.52 : c1[i] * static_cast<double>(s) * g[s];
+        const double per_wave = two ? 0.49 : c1[i] * static_cast<double>(s) * g[s];
         const double t = rounds * 64.0 * k * per_wave;
         if (i == 0 || t < best_t) {
             best_t = t;

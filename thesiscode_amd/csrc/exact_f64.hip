@@ -363,6 +363,28 @@ __device__ __forceinline__ int min_quarters(int x) {
     return step(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
 }
 
+// Maximum of N integers, three at a time.  The instruction is written out (as
+// in dmax) so that the scheduler keeps the tree where the cells are, instead of
+// stretching their registers' lives.
+__device__ __forceinline__ int imax3(int a, int b, int c) {
+    int r;
+    asm("v_max3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+template <int N, int K0 = 0>
+__device__ __forceinline__ int hi_max_tree(const int* h) {
+    if constexpr (N == 1) {
+        return h[K0];
+    } else if constexpr (N == 2) {
+        return imax3(h[K0], h[K0 + 1], h[K0 + 1]);
+    } else if constexpr (N == 3) {
+        return imax3(h[K0], h[K0 + 1], h[K0 + 2]);
+    } else {
+        constexpr int A = (N + 2) / 3, B = (N - A + 1) / 2;
+        return imax3(hi_max_tree<A, K0>(h), hi_max_tree<B, K0 + A>(h), hi_max_tree<N - A - B, K0 + A + B>(h));
+    }
+}
+
 // Maximum of N consecutive cells starting at cell K0 (pairwise tree).
 template <int N, int K0 = 0>
 __device__ __forceinline__ double max_tree(const double* sc) {
@@ -862,13 +884,23 @@ __device__ __forceinline__ double rotate_up_f64(double src) {
 // the keys rank exact ties, and the key names the K rows from that partial's
 // first (index 64: the K rows ending at j+BW-1).
 //
-// A step has one exec-masked region (each costs a branch): the column pairs
-// are written by every lane.
+// Column filter.  Once the profile fills, a band's cells rarely beat a
+// column's current result (at n = 2^22 a column has a candidate in 0.06% of
+// the windows), so a step first tests its cells against that result, read by
+// a scalar load one step ahead: the maximum of the cells' high words (three
+// integer maxima per three cells) against the result's high word.  Only when
+// some lane passes does the step build its pairs (six FP64 maxima) and write
+// them, marking the column in a window mask; the flush reduces marked columns
+// only, and most windows reduce none.
+//
+// A step has one exec-masked region (each costs a branch) and one uniform
+// branch (the column filter).
 template <int K>
 __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec,
                                               const WalkRec* __restrict__ rowrec, const double* qt,
                                               unsigned long long* rowval, unsigned long long* rowkey,
                                               unsigned long long* colval, unsigned long long* colkey,
+                                              const unsigned long long* __restrict__ colthr,
                                               unsigned int* counter, int64_t row_off, int64_t col_off,
                                               int blocklen, int tl, int ib, int nbands) {
     constexpr int F = kFlush;
@@ -892,39 +924,47 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
     const double QNAN = __builtin_nan("");
     const int lid0 = lane * K;
     WalkRec* const feed = &s_feed[wv][0];
+    // the high words of the column results (little-endian: word 1 of each)
+    const uint32_t* const colthr_hi = reinterpret_cast<const uint32_t*>(colthr) + 1;
 
     // Flush columns [jstart, jstart+cnt) and the rows that left at those
     // columns, as walk_body's flush does; lane (cs, qq) reduces a quarter of
     // column cs's partials and the 65th.
-    auto flush = [&](int jstart, int cnt, int dbase, uint64_t& pc, uint64_t& pr, auto reload)
+    auto flush = [&](int jstart, int cnt, int dbase, uint64_t& pc, uint64_t& pr, unsigned colmask, auto reload)
         __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
         const int cs = lane & (F - 1), qq = lane / F;
-        const double2* const pv = &s_cv[wv][cs][qq * Q];
-        double v[Q + 1];
-#pragma unroll
-        for (int u = 0; u < Q; ++u) {
-            const double2 pr2 = pv[u];
-            v[u] = dmax(pr2.x, pr2.y);
-            if (u == 0) v[0] = (qq == 0) ? pr2.x : v[0];
-        }
-        v[Q] = s_cv[wv][cs][0].y;
-        const double thr = unord_f64((pc >> ib) << ib);
-        double mx = max_tree<Q + 1>(v);
-        const bool cand = (mx >= thr) && (cs < cnt);
         const int jj = jstart + lane;
         uint64_t oc = 0;
         ScoreId b{NEG, -1};
-        if (__builtin_amdgcn_ballot_w64(cand) != 0) {
-            mx = max_quarters(mx);
-            int id = (v[Q] == mx) ? kWave : kWave + 1;
+        // only the columns whose step wrote its partials (colmask) can hold a
+        // candidate; the others' LDS rows are stale
+        if (colmask != 0) {
+            const double2* const pv = &s_cv[wv][cs][qq * Q];
+            double v[Q + 1];
 #pragma unroll
-            for (int u = Q - 1; u >= 0; --u) id = (v[u] == mx) ? qq * Q + u : id;
-            id = min_quarters(id);
-            b = ScoreId{mx, id <= kWave ? id : -1};
-            oc = ord_f64(mx);
+            for (int u = 0; u < Q; ++u) {
+                const double2 pr2 = pv[u];
+                v[u] = dmax(pr2.x, pr2.y);
+                if (u == 0) v[0] = (qq == 0) ? pr2.x : v[0];
+            }
+            v[Q] = s_cv[wv][cs][0].y;
+            const double thr = unord_f64((pc >> ib) << ib);
+            double mx = max_tree<Q + 1>(v);
+            const bool cand = (mx >= thr) && (cs < cnt) && ((colmask >> cs) & 1u);
+            if (__builtin_amdgcn_ballot_w64(cand) != 0) {
+                mx = max_quarters(mx);
+                int id = (v[Q] == mx) ? kWave : kWave + 1;
+#pragma unroll
+                for (int u = Q - 1; u >= 0; --u) id = (v[u] == mx) ? qq * Q + u : id;
+                id = min_quarters(id);
+                b = ScoreId{mx, id <= kWave ? id : -1};
+                oc = ord_f64(mx);
+            }
         }
-        const bool do_col = (lane < cnt) & (b.id >= 0) & (b.v == b.v) & ((oc >> ib) >= (pc >> ib));
+        // (b of an unflagged column was taken from stale partials)
+        const bool do_col = (lane < cnt) & ((colmask >> cs) & 1u) & (b.id >= 0) & (b.v == b.v) &
+                            ((oc >> ib) >= (pc >> ib));
         const RowRes rr = s_rr[wv][lane & (F - 1)];
         const uint64_t orw = ord_f64(rr.v);
         // a row whose cells were all masked keeps -inf or the reset NaN
@@ -974,17 +1014,13 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
             tag[k] = -1;
         }
         WalkRec cur_c;             // the next step's column record
+        uint32_t cur_t;            // and the high word of its current result (ord)
+        unsigned colmask = 0;      // the window's steps that wrote their column partials
         const WalkRec* fw = feed;  // this window's entering rows
         int wl0 = 0;               // the wrap lane of the window's first column
         // this lane's rotated index, per half window (its wrap lane)
         int rot = 0;
         auto half = [&](int wl) __attribute__((always_inline)) { rot = (lane - wl) & (kWave - 1); };
-        // A step's column pair is written by the next step, after its wait
-        // for the scalar records (as in walk_body).
-        double pav = NEG, pbv = NEG;
-        auto put_pair = [&](int w, int r) __attribute__((always_inline)) {
-            s_cv[wv][w][r] = make_double2(pav, pbv);
-        };
         auto step = [&](int c0, auto w_tag, auto tail_tag) __attribute__((always_inline)) -> bool {
             constexpr int W = decltype(w_tag)::value;
             constexpr int S = W % K;             // the wrap slot: its row leaves after this step
@@ -998,13 +1034,11 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
             // values) covers every scalar and LDS operation in flight; the
             // next step's record is loaded after it.
             const WalkRec c = cur_c;
-            asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu) : "memory");
+            const uint32_t th = cur_t;
+            asm volatile("" ::"s"(c.a), "s"(c.am), "s"(c.s), "s"(c.mu), "s"(th) : "memory");
             cur_c = colrec[j + 1];
-            if constexpr (W > 0) {
-                const int rprev = rot;  // the last step's wrap lane
-                if constexpr (S == 0) half(wl0 + W / K);
-                put_pair(W - 1, rprev);
-            }
+            cur_t = colthr_hi[2 * (j + 1)];  // j + 1 <= L <= tl: the self-join's P > W entries
+            if constexpr (W > 0 && S == 0) half(wl0 + W / K);
             __builtin_amdgcn_sched_barrier(0);
             const double ahj = c.a, ahjm = c.am, shj = c.s, mhj = c.mu;
             double sc[K];
@@ -1046,8 +1080,25 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
             qr[tl7] = rotate_up_f64(qr[tl7]);
 #pragma unroll
             for (int t = 1; t < K; ++t) asm volatile("" ::"v"(qr[t]));
-            pav = max_tree<K - S, S>(sc);
-            pbv = (S == 0) ? NEG : max_tree<(S > 0 ? S : 1), 0>(sc);
+            // The column's partials are needed only if some cell can reach
+            // the column's current result.  A test on the cells' high words,
+            // as signed integers, passes every such cell: for positive scores
+            // they order as the scores do (ties included, as the result keys
+            // truncate only low-word bits), a negative score is a negative
+            // integer, and a masked cell's NaN passes (the partials then take
+            // care of it).  A result that is not positive (none yet, or the
+            // -1 it starts from) passes every cell.
+            const int sgn = static_cast<int>(th) >> 31;  // -1: a positive score
+            const int thi = (static_cast<int>(th & 0x7FFFFFFFu) & sgn) | (~sgn & INT_MIN);
+            int hv[K];
+#pragma unroll
+            for (int s = 0; s < K; ++s) hv[s] = __double2hiint(sc[s]);
+            if (__builtin_amdgcn_ballot_w64(hi_max_tree<K>(hv) >= thi) != 0) {
+                const double pav = max_tree<K - S, S>(sc);
+                const double pbv = (S == 0) ? NEG : max_tree<(S > 0 ? S : 1), 0>(sc);
+                s_cv[wv][W][rot] = make_double2(pav, pbv);
+                colmask |= 1u << W;
+            }
             return true;
         };
         // rows dbase + c0 + BW + x, x < F (padding records past the
@@ -1076,8 +1127,6 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
             [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
                 ((go = go && step(c0, std::integral_constant<int, W>{}, tail_tag)), ...);
             }(std::make_integer_sequence<int, F>{});
-            const int wlast = decltype(tail_tag)::value ? min(F, L - c0) - 1 : F - 1;
-            put_pair(wlast, rot);
         };
         auto current = [&](int c0, uint64_t& pc, uint64_t& pr) __attribute__((always_inline)) {
             const int jj = min(c0 + (lane & (F - 1)), L - 1);
@@ -1092,18 +1141,21 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
         stage(0, 0);
         __builtin_amdgcn_wave_barrier();
         cur_c = colrec[0];
+        cur_t = colthr_hi[0];
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         for (; c0 + F <= L; c0 += F) {
+            colmask = 0;
             window(c0, std::false_type{});
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            flush(c0, F, dbase, pc, pr, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
+            flush(c0, F, dbase, pc, pr, colmask, [&](uint64_t& a, uint64_t& b) __attribute__((always_inline)) {
                 current(c0 + F, a, b);
             });
         }
         if (c0 < L) {
+            colmask = 0;
             window(c0, std::true_type{});
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
+            flush(c0, L - c0, dbase, pc, pr, colmask, [](uint64_t&, uint64_t&) {});
         }
     }
 }
@@ -1144,10 +1196,10 @@ __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(2))) voi
 // self-join only (no final-QT write-back).
 template <int K>
 __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(2))) void k_walk_f64_rot(
-    MP_WALK_PARAMS) {
+    MP_WALK_PARAMS, const unsigned long long* __restrict__ colthr) {
     (void)m;
-    walk_rot_body<K>(colrec, rowrec, qt, rowval, rowkey, colval, colkey, counter, row_off, col_off, blocklen,
-                     tl, ib, nbands);
+    walk_rot_body<K>(colrec, rowrec, qt, rowval, rowkey, colval, colkey, colthr, counter, row_off, col_off,
+                     blocklen, tl, ib, nbands);
 }
 #undef MP_WALK_PARAMS
 #undef MP_WALK_ARGS
@@ -1303,10 +1355,12 @@ int launch_walk_f64(const WalkF64Launch& a, hipStream_t st) {
                                a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib, nbands);       \
         break;
     if (a.band_k == 8 && a.form == 3) {
-        if (a.writeback) return MP_ERR_BAD_ARG;
+        // the self-join only: one block of every diagonal, and column results
+        // of P > tl entries (the walker reads the one after its last column)
+        if (a.writeback || a.blocklen != a.tl) return MP_ERR_BAD_ARG;
         hipLaunchKernelGGL((k_walk_f64_rot<8>), grid, block, 0, st, a.colrec, a.rowrec, a.qt, a.rowval, a.rowkey,
                            a.colval, a.colkey, a.counter, a.row_off, a.col_off, a.blocklen, a.tl, a.m, a.ib,
-                           nbands);
+                           nbands, static_cast<const unsigned long long*>(a.colval));
     } else if (a.band_k == 8 && a.form == 2) {
         switch (a.band_k) { MP_WALK_CASE(8, k_walk_f64_pair) }
     } else {
