@@ -1084,10 +1084,11 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
             // the column's current result.  A test on the cells' high words,
             // as signed integers, passes every such cell: for positive scores
             // they order as the scores do (ties included, as the result keys
-            // truncate only low-word bits), a negative score is a negative
-            // integer, and a masked cell's NaN passes (the partials then take
-            // care of it).  A result that is not positive (none yet, or the
-            // -1 it starts from) passes every cell.
+            // truncate only low-word bits: ib <= 32 for P <= 2^30), a
+            // negative score is a negative integer, and a masked cell's NaN
+            // passes (the partials then take care of it).  A result that is
+            // not positive (none yet, or the -1 it starts from) passes every
+            // cell.
             const int sgn = static_cast<int>(th) >> 31;  // -1: a positive score
             const int thi = (static_cast<int>(th & 0x7FFFFFFFu) & sgn) | (~sgn & INT_MIN);
             int hv[K];
