@@ -218,6 +218,22 @@ def test_block_drop_in_separate_row_and_column_arrays():
         assert np.array_equal(a, b)
 
 
+def test_config2_rotating_form_equals_ring_form():
+    # n = 2^20 random walk (seed 1), m = 256: the rotating form (column filter,
+    # single-band items) and the LDS-ring form walk the same cells in different
+    # orders and lanes, so their values must agree bit for bit at every entry;
+    # indices may differ only at near ties
+    A = tc.random_walk(1, 1 << 20)
+    m = 256
+    print("cfg2 ring", flush=True)
+    p1, i1, t1 = tc.selfjoin_f64(A, m, band_k=8, walk_form=1)
+    print("cfg2 rotating", flush=True)
+    p3, i3, t3 = tc.selfjoin_f64(A, m, band_k=8, walk_form=3)
+    assert t1["walk_form"] == 1 and t3["walk_form"] == 3
+    assert np.array_equal(p1, p3)
+    assert_indices_match(A, m, p1, i1, i3, abs_tol=_tie_tol(m), min_frac=0.999)
+
+
 def test_config2_full_size_properties():
     # n = 2^20 random walk (seed 1), m = 256: too large for the oracle in a test;
     # checked by recomputation, exclusion, bounds, row brute force and determinism
