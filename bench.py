@@ -157,7 +157,7 @@ def run_batch_bench(steps: int, warmup: int, tune: bool):
             P = int(b.out_offsets[-1])
             d_p = dev.alloc(8 * P)
             d_i = dev.alloc(8 * P)
-            tuned = b.tune(d_ts, d_p, d_i, max_tile_len=8192) if tune else None
+            tuned = b.tune(d_ts, d_p, d_i) if tune else None
             for _ in range(warmup):
                 b.run(d_ts, d_p, d_i, use_graph=True)
             dev.sync()

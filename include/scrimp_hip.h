@@ -185,7 +185,8 @@ int mp_eval_diag_block_triangle_f64(double prof_colmin[], int64_t idx_colmin[],
  * each series is windows[s]/4 unless opts->exclusion >= 0.  With use_graph
  * the whole batch (statistics, tiles, unpack) is captured once in a hipGraph
  * and replayed while the pointers stay the same.  mp_batch_tune times band
- * widths 4/8 and tile lengths 256..max_tile_len (<= 0: 8192) on the real
+ * widths 4/8 and tile lengths 256..max_tile_len (<= 0: the longest automatic
+ * one, 32768 with FP64 accumulation and 1024 with FP32) on the real
  * batch, keeps the fastest and records it in an on-disk cache that later
  * mp_batch_create calls consult ($SCRIMP_HIP_TUNE_CACHE, default
  * ~/.cache/scrimp_hip/).

@@ -220,7 +220,7 @@ class Batch:
                                  1 if use_graph else 0, C.byref(tm)), "mp_batch_run")
         return tm.as_dict()
 
-    def tune(self, d_ts: int, d_prof: int, d_idx: int, max_tile_len: int = 8192) -> dict:
+    def tune(self, d_ts: int, d_prof: int, d_idx: int, max_tile_len: int = 0) -> dict:
         tm = Timing()
         check(lib().mp_batch_tune(self._h, C.c_void_p(d_ts), C.c_void_p(d_prof), C.c_void_p(d_idx),
                                   max_tile_len, C.byref(tm)), "mp_batch_tune")

@@ -529,14 +529,15 @@ int mp_prescrimp_f64(const double* ts, int64_t n, int32_t m, double stride, doub
 // FP32-reseeded self-join (a batch of one series)
 // ---------------------------------------------------------------------------
 // Columns per tile (the re-seed interval): the longest of max_auto_tile_len(accum64)
-// ... 512 that still gives every SIMD of the device 4 tiles' worth of waves.
-// Longer tiles amortise the FP64 seed (m FMAs per diagonal) over more cells; the
-// cap keeps an automatic choice inside the accuracy the accumulation mode holds.
+// ... 512 that still gives every SIMD of the device 4 tiles' worth of waves
+// (above kShortTileMax: 16 tiles per wave, long_tile_target).  Longer tiles
+// amortise the FP64 seed (m FMAs per diagonal) over more cells; the cap keeps an
+// automatic choice inside the accuracy the accumulation mode holds.
 static int default_tile_len(int64_t W, int band_k, int cus, int requested, int accum64) {
     if (requested > 0) return requested;
     const int64_t bw = 64LL * band_k;
-    const int64_t target = 16LL * cus;
     for (int T = mp::max_auto_tile_len(accum64); T >= 512; T >>= 1) {
+        const int64_t target = T > mp::kShortTileMax ? mp::long_tile_target(cus) : 16LL * cus;
         int64_t nt = 0;
         for (int64_t d = 0; d < W; d += bw) nt += (W - d + T - 1) / T;
         if (nt >= target) return T;

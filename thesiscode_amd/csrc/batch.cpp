@@ -304,9 +304,11 @@ int mp_batch_create(mp_ctx* ctx, int32_t nseries, const int64_t* lengths, const 
         if (rc != MP_OK) return fail(rc);
     } else {
         // the longest tile (max_auto_tile_len ... 512) that still gives every
-        // SIMD four tiles' worth of waves, as for one series
-        const int64_t target = 16LL * std::max(1, b->cus);
+        // SIMD four tiles' worth of waves (above kShortTileMax 16 tiles per
+        // wave), as for one series
         for (int T = mp::max_auto_tile_len(b->accum64);; T >>= 1) {
+            const int64_t target =
+                T > mp::kShortTileMax ? mp::long_tile_target(b->cus) : 16LL * std::max(1, b->cus);
             rc = b->plan.build(nseries, lengths, windows, b->excl, bk, T);
             if (rc != MP_OK) return fail(rc);
             tl = T;
@@ -402,10 +404,10 @@ int mp_batch_tune(mp_batch* b, const double* d_ts, double* d_profile, int64_t* d
     if (!b || !d_ts || !d_profile || !d_index) return MP_ERR_NULL;
     // the tuner ranks by speed only among tile lengths whose accuracy the
     // accumulation mode holds (max_auto_tile_len)
-    if (max_tile_len <= 0) max_tile_len = 8192;
+    if (max_tile_len <= 0) max_tile_len = mp::max_auto_tile_len(b->accum64);
     max_tile_len = std::min(max_tile_len, mp::max_auto_tile_len(b->accum64));
     const int bks[] = {4, 8};
-    const int tls[] = {256, 512, 1024, 2048, 4096, 8192};
+    const int tls[] = {256, 512, 1024, 2048, 4096, 8192, 16384, 32768};
     double best_rate = -1.0;
     int best_bk = b->plan.band_k, best_tl = b->plan.tile_len;
     for (int bk : bks) {

@@ -67,12 +67,19 @@ __host__ __device__ inline int64_t band_tiles(int64_t L, int band_k, int T) {
 
 // Longest re-seed interval (tile length) that the library, the batch autotuner
 // and the server pick by themselves.  With FP64 accumulation every interval up
-// to 8192 keeps the profile within 1e-4 of FP64 (n = 2^22, m = 1024 near-periodic
-// series: 4.3e-5 at every B).  FP32 accumulation drifts with the interval: on
-// random walks it stays within 1e-4 up to B = 1024 (6.5e-5) and leaves it from
-// B = 4096 on (1.5e-4), so automatic choices stop at 1024 there; a caller that
-// asks for a longer interval explicitly still gets it.
-inline int max_auto_tile_len(int accum64) { return accum64 ? 8192 : 1024; }
+// to 65536 keeps the profile within 1e-4 of FP64 (n = 2^22, m = 1024 near-periodic
+// series: 1.9e-5 at every B from 256 to 65536; n = 2^20, m = 256 random walk:
+// 2.0e-5 up to 16384, 3.2e-5 at 32768).  FP32 accumulation drifts with the
+// interval: on random walks it stays within 1e-4 up to B = 1024 (6.5e-5) and
+// leaves it from B = 4096 on (1.5e-4), so automatic choices stop at 1024 there;
+// a caller that asks for a longer interval explicitly still gets it.
+inline int max_auto_tile_len(int accum64) { return accum64 ? 32768 : 1024; }
+// Tile lengths above kShortTileMax are taken only where they still leave 16
+// tiles for each resident wave (16 per CU): longer tiles save seeds (2.7% of the
+// kernel at n = 2^22, m = 1024 from 8192 to 32768), but too few of them leave
+// waves idle at the end (n = 2^20, m = 256: 78 ms at 8192 or 16384, 81 ms at 32768).
+constexpr int kShortTileMax = 8192;
+inline int64_t long_tile_target(int cus) { return 256LL * (cus > 0 ? cus : 1); }
 
 // Window statistics are computed in chunks of kStatChunk windows of one series;
 // each chunk restarts its sums at its first window, so the statistics of a
