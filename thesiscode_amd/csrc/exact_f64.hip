@@ -887,8 +887,10 @@ __device__ __forceinline__ double rotate_up_f64(double src) {
 // Column filter.  Once the profile fills, a band's cells rarely beat a
 // column's current result (at n = 2^22 a column has a candidate in 0.06% of
 // the windows), so a step first tests its cells against that result, read by
-// a scalar load one step ahead: the maximum of the cells' high words (three
-// integer maxima per three cells) against the result's high word.  Only when
+// a scalar load one step ahead through colthr (colval again, read-only; the
+// scalar cache may return an older value, which is lower, as results only
+// grow): the maximum of the cells' high words (v_max3_i32) against the
+// result's high word.  Only when
 // some lane passes does the step build its pairs (six FP64 maxima) and write
 // them, marking the column in a window mask; the flush reduces marked columns
 // only, and most windows reduce none.
