@@ -60,6 +60,22 @@ def test_reseed_interval_variants(tile_len, accum64):
     assert erru.max() <= 1e-2
 
 
+@pytest.mark.parametrize("tile_len", [16384, 32768])
+def test_long_reseed_intervals_with_fp64_accumulation(tile_len):
+    # the automatic intervals above 8192 (FP64 accumulation only): bands of up
+    # to 2^17 columns cut into several long tiles and an edge tile
+    A, _ = tc.precision_series(6, 1 << 17, 256)
+    m = 256
+    p64, i64, _ = tc.selfjoin_f64(A, m)
+    p, i, tm = tc.selfjoin_f32r(A, m, tile_len=tile_len)
+    assert tm["tile_len"] == tile_len
+    err = rel_err(p, p64, m)
+    print(f"B={tile_len} max_rel={err.max():.3e} mean_rel={err.mean():.3e} idx_equal={np.mean(i == i64):.5f}")
+    assert err.max() <= F64_ACC_MAX_REL
+    assert err.mean() <= 1e-6
+    assert_indices_match(A, m, p64, i64, i, abs_tol=0.0, rel_tol=1e-4, min_frac=0.999)
+
+
 def test_config2_full_size_within_tolerance_of_fp64():
     # n = 2^20 random walk (seed 1), m = 256: the measured workload, default tiles
     A = tc.random_walk(1, 1 << 20)
@@ -67,6 +83,9 @@ def test_config2_full_size_within_tolerance_of_fp64():
     p64, i64, _ = tc.selfjoin_f64(A, m)
     for acc in (True, False):
         p, i, tm = tc.selfjoin_f32r(A, m, accum64=acc)
+        # the automatic interval: the longest that leaves 16 tiles per wave
+        # (FP64 accumulation) or the FP32-accumulation cap
+        assert tm["tile_len"] == (16384 if acc else 1024)
         err = rel_err(p, p64, m)
         print(f"accum64={acc} T={tm['tile_len']} max_rel={err.max():.3e} mean_rel={err.mean():.3e} "
               f"idx_equal={np.mean(i == i64):.5f}")
