@@ -548,7 +548,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every GPU box here has one GPU, and the library has no "
-              "multi-GPU path (a self-join would shard by band pairs, one rank per GPU, merged by one "
+              "multi-GPU path (a self-join would shard by bands, one rank per GPU, merged by one "
               "max all-reduce of the packed keys); run with --gpus 1", file=sys.stderr)
         return 2
 
