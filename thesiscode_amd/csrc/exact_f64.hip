@@ -363,6 +363,20 @@ __device__ __forceinline__ int min_quarters(int x) {
     return step(x, [](unsigned v) { return __builtin_amdgcn_permlane16_swap(v, v, false, false); });
 }
 
+// dmax() and min() over lanes l and l^32, in every lane (one gfx950 row swap).
+__device__ __forceinline__ double max_halves(double x) {
+    const auto lo = __builtin_amdgcn_permlane32_swap(static_cast<unsigned>(__double2loint(x)),
+                                                     static_cast<unsigned>(__double2loint(x)), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(static_cast<unsigned>(__double2hiint(x)),
+                                                     static_cast<unsigned>(__double2hiint(x)), false, false);
+    return dmax(__hiloint2double(static_cast<int>(hi[0]), static_cast<int>(lo[0])),
+                __hiloint2double(static_cast<int>(hi[1]), static_cast<int>(lo[1])));
+}
+__device__ __forceinline__ int min_halves(int x) {
+    const auto s = __builtin_amdgcn_permlane32_swap(static_cast<unsigned>(x), static_cast<unsigned>(x), false, false);
+    return min(static_cast<int>(s[0]), static_cast<int>(s[1]));
+}
+
 // Maximum of N integers, three at a time.  The instruction is written out (as
 // in dmax) so that the scheduler keeps the tree where the cells are, instead of
 // stretching their registers' lives.
@@ -905,18 +919,23 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
                                               const unsigned long long* __restrict__ colthr,
                                               unsigned int* counter, int64_t row_off, int64_t col_off,
                                               int blocklen, int tl, int ib, int nbands) {
-    constexpr int F = kFlush;
+    // 32 steps per flush window (the other walkers use 16): with the column
+    // filter most flushes only merge rows, so fewer of them save their fixed
+    // cost, and one partial per lane (below) keeps the LDS for two groups per CU
+    constexpr int F = 2 * kFlush;
     constexpr int BW = kWave * K;
     // a window starts at wrap slot 0 and its positions never pass the wave's end
     static_assert(F % K == 0 && BW % F == 0, "window and band geometry");
-    constexpr int Q = kWave / 4;  // pairs per flush lane
+    static_assert(F == kWave / 2, "the flush reduces each column in two halves of the wave");
+    constexpr int Q = kWave / 2;  // partials per flush lane
     struct RowRes {
         double v;
         int tag, pad;
     };
-    // column pairs, rows padded by one pair: the 16 flush lanes reading one
-    // index of 16 columns then fall on distinct banks
-    __shared__ double2 s_cv[kWavesPerGroup][F][kWave + 1];
+    // Column partials: index r = (l - wl) mod 64 holds lane l's maximum, except
+    // that the wrap lane (r = 0) keeps its two halves apart, the second at
+    // index 64.  Rows of 66 doubles.
+    __shared__ double s_cv[kWavesPerGroup][F][kWave + 2];
     __shared__ RowRes s_rr[kWavesPerGroup][F];  // the leaving row's result per step
     __shared__ __align__(16) WalkRec s_feed[kWavesPerGroup][2 * F];
 
@@ -930,8 +949,8 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
     const uint32_t* const colthr_hi = reinterpret_cast<const uint32_t*>(colthr) + 1;
 
     // Flush columns [jstart, jstart+cnt) and the rows that left at those
-    // columns, as walk_body's flush does; lane (cs, qq) reduces a quarter of
-    // column cs's partials and the 65th.
+    // columns, as walk_body's flush does; lane (cs, qq) reduces half of column
+    // cs's partials and the 65th.
     auto flush = [&](int jstart, int cnt, int dbase, uint64_t& pc, uint64_t& pr, unsigned colmask, auto reload)
         __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
@@ -942,24 +961,20 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
         // only the columns whose step wrote its partials (colmask) can hold a
         // candidate; the others' LDS rows are stale
         if (colmask != 0) {
-            const double2* const pv = &s_cv[wv][cs][qq * Q];
+            const double* const pv = &s_cv[wv][cs][qq * Q];
             double v[Q + 1];
 #pragma unroll
-            for (int u = 0; u < Q; ++u) {
-                const double2 pr2 = pv[u];
-                v[u] = dmax(pr2.x, pr2.y);
-                if (u == 0) v[0] = (qq == 0) ? pr2.x : v[0];
-            }
-            v[Q] = s_cv[wv][cs][0].y;
+            for (int u = 0; u < Q; ++u) v[u] = pv[u];
+            v[Q] = s_cv[wv][cs][kWave];
             const double thr = unord_f64((pc >> ib) << ib);
             double mx = max_tree<Q + 1>(v);
             const bool cand = (mx >= thr) && (cs < cnt) && ((colmask >> cs) & 1u);
             if (__builtin_amdgcn_ballot_w64(cand) != 0) {
-                mx = max_quarters(mx);
+                mx = max_halves(mx);
                 int id = (v[Q] == mx) ? kWave : kWave + 1;
 #pragma unroll
                 for (int u = Q - 1; u >= 0; --u) id = (v[u] == mx) ? qq * Q + u : id;
-                id = min_quarters(id);
+                id = min_halves(id);
                 b = ScoreId{mx, id <= kWave ? id : -1};
                 oc = ord_f64(mx);
             }
@@ -1099,7 +1114,8 @@ __device__ __forceinline__ void walk_rot_body(const WalkRec* __restrict__ colrec
             if (__builtin_amdgcn_ballot_w64(hi_max_tree<K>(hv) >= thi) != 0) {
                 const double pav = max_tree<K - S, S>(sc);
                 const double pbv = (S == 0) ? NEG : max_tree<(S > 0 ? S : 1), 0>(sc);
-                s_cv[wv][W][rot] = make_double2(pav, pbv);
+                s_cv[wv][W][rot] = (rot == 0) ? pav : dmax(pav, pbv);
+                if (rot == 0) s_cv[wv][W][kWave] = pbv;
                 colmask |= 1u << W;
             }
             return true;
