@@ -890,13 +890,13 @@ __device__ __forceinline__ double rotate_up_f64(double src) {
 // S = j mod K) holds rows j .. j+K-1-S in slots S .. K-1 and the band's last
 // rows j+BW-S .. j+BW-1 in slots 0 .. S-1; every other lane holds K
 // consecutive rows, those of lane wl + r starting at j + K r - S.  Every lane
-// writes the pair (maximum over slots S .. K-1, maximum over slots 0 .. S-1)
-// at index r = (l - wl) mod 64, so the pairs of a column lie in ascending row
-// order.  The flush takes pair r's maximum for r > 0, the first half of pair 0
-// as partial 0 and its second half as a 65th partial (index 64, the largest
-// rows): its smallest index holding the maximum is then the smallest row, as
-// the keys rank exact ties, and the key names the K rows from that partial's
-// first (index 64: the K rows ending at j+BW-1).
+// writes its maximum at index r = (l - wl) mod 64, so the partials of a column
+// lie in ascending row order, except that the wrap lane writes the maximum
+// over slots S .. K-1 at index 0 and the one over slots 0 .. S-1 (the band's
+// largest rows) as a 65th partial at index 64: the flush's smallest index
+// holding the maximum is then the smallest row, as the keys rank exact ties,
+// and the key names the K rows from that partial's first (index 64: the K rows
+// ending at j+BW-1).
 //
 // Column filter.  Once the profile fills, a band's cells rarely beat a
 // column's current result (at n = 2^22 a column has a candidate in 0.06% of
