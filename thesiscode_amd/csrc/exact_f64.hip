@@ -430,14 +430,20 @@ __device__ __forceinline__ void walk_body(
     double* qt, unsigned long long* rowval, unsigned long long* rowkey,
     unsigned long long* colval, unsigned long long* colkey, unsigned int* counter,
     int64_t row_off, int64_t col_off, int blocklen, int tl, int m, int ib, int nbands) {
-    constexpr int F = kFlush;
+    // the ring form (one wave per SIMD, latency-bound) flushes every 32 steps,
+    // the others every 16
+    constexpr int F = RING ? 2 * kFlush : kFlush;
+    constexpr int NQ = kWave / F;   // parts of a column's partials, one per flush lane
+    constexpr int PQ = kWave / NQ;  // partials per flush lane
+    static_assert(NQ == 2 || NQ == 4, "a column's partials reduce in halves or quarters");
     constexpr int BW = kWave * K;
     static_assert(F % K == 0, "flush period must be a multiple of K");
     static_assert(!(RING && FEEDLDS), "the ring form reads every row from LDS");
     // Column partials, one row of 64 per step, padded by one entry: a flush
-    // lane reads 16 consecutive entries of one row (immediate offsets), and
-    // the 16 lanes of an LDS phase then fall on distinct banks.  Without the
-    // ring, 42 KB of LDS per group, so that 3 groups still fit in a CU.
+    // lane reads PQ consecutive entries of one row (immediate offsets), and
+    // the lanes of an LDS phase then fall on distinct banks.  Without the
+    // ring, 42 KB of LDS per group, so that 3 groups still fit in a CU; with
+    // it (32-step windows), 158 KB.
     __shared__ double s_cv[kWavesPerGroup][F][kWave + 1];
     // [0, F): rows leaving lane 0; lane l > 0 writes scratch slot F + l + step
     __shared__ double s_rv[kWavesPerGroup][2 * F + kWave];
@@ -501,17 +507,17 @@ __device__ __forceinline__ void walk_body(
     auto flush = [&](int jstart, int cnt, int dbase, uint64_t& pc, uint64_t& pr, auto reload)
         __attribute__((always_inline)) {
         __builtin_amdgcn_wave_barrier();
-        // Lane (cs, qq) holds a quarter of column cs's 64 partials (those of
-        // lanes 16 qq .. 16 qq + 15).  A column result passes the filter iff
+        // Lane (cs, qq) holds part qq of column cs's 64 partials (those of
+        // lanes PQ qq .. PQ qq + PQ - 1).  A column result passes the filter iff
         // its score reaches thr, the score of the truncated current value (the
         // order map is monotone), so a column can pass only if one of its
         // partials does.  The reduction runs only when some partial of the
         // window passes.  For cs >= cnt the quarter is stale and unused.
         const int cs = lane & (F - 1), qq = lane / F;
-        const double* const pv = &s_cv[wv][cs][qq * (kWave / 4)];
-        double v[kWave / 4];
+        const double* const pv = &s_cv[wv][cs][qq * PQ];
+        double v[PQ];
 #pragma unroll
-        for (int u = 0; u < kWave / 4; ++u) v[u] = pv[u];
+        for (int u = 0; u < PQ; ++u) v[u] = pv[u];
         if constexpr (RING) {  // the filter values, read behind the partials: one LDS wait for both
             __builtin_amdgcn_sched_barrier(0);
             pc = filt[lane & (F - 1)];
@@ -520,19 +526,19 @@ __device__ __forceinline__ void walk_body(
         const double thr = unord_f64((pc >> ib) << ib);
         // the quarter's exact maximum (a NaN partial is a masked cell, and
         // v_max_f64 returns the other operand; all NaN: NaN, which fails)
-        double mx = max_tree<kWave / 4>(v);
+        double mx = max_tree<PQ>(v);
         const bool cand = (mx >= thr) && (cs < cnt);
         const int jj = jstart + lane;
         uint64_t oc = 0;
         ScoreId b{NEG, -1};
         if (__builtin_amdgcn_ballot_w64(cand) != 0) {
             // the column's maximum, then the smallest lane holding it: this
-            // quarter's first, then the smallest over the quarters
-            mx = max_quarters(mx);
+            // part's first, then the smallest over the parts
+            mx = (NQ == 4) ? max_quarters(mx) : max_halves(mx);
             int id = kWave;
 #pragma unroll
-            for (int u = kWave / 4 - 1; u >= 0; --u) id = (v[u] == mx) ? qq * (kWave / 4) + u : id;
-            id = min_quarters(id);
+            for (int u = PQ - 1; u >= 0; --u) id = (v[u] == mx) ? qq * PQ + u : id;
+            id = (NQ == 4) ? min_quarters(id) : min_halves(id);
             b = ScoreId{mx, id < kWave ? id : -1};
             oc = ord_f64(mx);
         }

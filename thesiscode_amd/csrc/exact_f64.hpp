@@ -15,8 +15,10 @@ namespace mp {
 struct alignas(32) WalkRec {
     double a, am, s, mu;
 };
-// padding records needed past the last row: one band of the widest walker
-constexpr int kWalkRecPad = 64 * 8 + 64;
+// padding records needed past the last row: one band of the widest walker and
+// three of its 32-step windows (the ring form stages rows up to BW + 3F - 2
+// past its last column)
+constexpr int kWalkRecPad = 64 * 8 + 3 * 32;
 
 // Device bytes the exact self-join carves from its scratch for n points, P
 // windows and W walked diagonals (each buffer 256-byte aligned): A (n+64),
