@@ -112,7 +112,74 @@ def test_rotating_form_results_that_stay_negative():
 
 def test_walk_form_out_of_range_is_refused():
     with pytest.raises(tc.ScrimpError):
+        tc.selfjoin_f64(tc.random_walk(1, 2000), 32, band_k=8, walk_form=5)
+    with pytest.raises(tc.ScrimpError):  # column segments: band widths 1, 2 and 4 only
         tc.selfjoin_f64(tc.random_walk(1, 2000), 32, band_k=8, walk_form=4)
+
+
+def _seg_items(W, band_k, seg_cols=2048):
+    bw = 64 * band_k
+    r = seg_cols // bw
+    return sum(-(-(W - b * bw) // (r * bw)) for b in range(-(-W // bw)))
+
+
+@pytest.mark.parametrize("band_k", [1, 2, 4])
+@pytest.mark.parametrize("n,m,seed", [(20000, 64, 31), (65536, 400, 0)])
+def test_column_segments_bit_exact(band_k, n, m, seed):
+    # walk_form 4: every band cut into 2048-column segments, walked by waves of
+    # their own; a segment after the first starts from the values a pre-pass
+    # advanced along its diagonals, and a row that a cut splits gets one result
+    # per part, merged by the result keys
+    A = tc.random_walk(seed, n)
+    key = (n, m, seed)
+    if key not in _FORM_CASES:
+        _FORM_CASES[key] = _oracle(A, m)
+    p_ref, i_ref = _FORM_CASES[key]
+    print(f"segments K={band_k} n={n} m={m}", flush=True)
+    p, i, tm = tc.selfjoin_f64(A, m, band_k=band_k, walk_form=4)
+    W = n - m - m // 4
+    assert tm["band_k"] == band_k and tm["walk_form"] == 4
+    assert tm["work_items"] == _seg_items(W, band_k) > -(-W // (64 * band_k))
+    assert np.array_equal(p, p_ref)
+    assert_indices_match(A, m, p_ref, i_ref, i, abs_tol=_tie_tol(m), min_frac=0.999)
+
+
+@pytest.mark.parametrize("W", [2047, 2048, 2049, 4096, 4097, 6200])
+def test_column_segments_at_segment_boundaries(W):
+    # bands whose length is a multiple of 2048, one past it (a one-column last
+    # segment) or short of it, at each band width; m = 64, exclusion 16
+    m = 64
+    n = W + m + m // 4
+    A = tc.random_walk(W, n)
+    p_ref, i_ref = _oracle(A, m)
+    for band_k in (1, 2, 4):
+        print(f"segments W={W} K={band_k}", flush=True)
+        p, i, tm = tc.selfjoin_f64(A, m, band_k=band_k, walk_form=4)
+        assert tm["work_items"] == _seg_items(W, band_k)
+        assert np.array_equal(p, p_ref), band_k
+        assert_indices_match(A, m, p_ref, i_ref, i, abs_tol=_tie_tol(m))
+
+
+def test_column_segments_with_ties_and_negative_results():
+    # a period-64 series repeated exactly: the diagonals at multiples of the
+    # period tie up to rounding; and the sine whose entries all keep the initial score, cut into segments
+    m = 32
+    base = tc.random_walk(9, 64)
+    A = np.tile(base, 80)
+    p_ref, i_ref = _oracle(A, m)
+    for band_k in (1, 4):
+        p, i, _ = tc.selfjoin_f64(A, m, band_k=band_k, walk_form=4)
+        assert np.array_equal(p, p_ref), band_k
+        # nearly every entry's partners tie beyond what the merge key resolves:
+        # any partner the path reports must be such a tie
+        assert_indices_match(A, m, p_ref, i_ref, i, abs_tol=_tie_tol(m), min_frac=0.0)
+    m, ez = 64, 16
+    n = 4500
+    t = np.arange(n, dtype=np.float64)
+    A = np.sin(2 * np.pi * t / (4 * ez)) + 1e-3 * tc.random_walk(78, n)
+    p_ref, i_ref = _oracle(A, m)
+    p, i, _ = tc.selfjoin_f64(A, m, band_k=4, walk_form=4)
+    assert np.array_equal(p, p_ref) and np.array_equal(i, i_ref)
 
 
 def test_config1_random_walk_bit_exact():

@@ -119,12 +119,19 @@ int geometry(int64_t n, int32_t m, const mp_opts* o, Geometry* g) {
 // more items than SIMDs, in its rotating form): 0.49 per wave.  Pick the K
 // that minimises it.  `form` (mp_opts.walk_form) picks the K = 8 form; 0
 // chooses the LDS ring at one wave per SIMD, the rotating form at two.
+//
+// Column segments (form 4, K = 4, up to W = kSegMaxW): the cells spread over
+// all SIMDs at three waves each, 0.75 * 3 * 0.57 ns per cell per wave, plus
+// the segment start values, a recurrence of kSegSeedNs per column along the
+// longest band; chosen when that beats the best whole-band time.
 struct WalkChoice {
     int band_k;
     int form;  // K = 8: the launch's form (1 LDS ring, 2 DPP rows at two waves per SIMD, 3 rotating diagonals)
 };
-WalkChoice pick_band_k_f64(const mp_ctx* c, int W, int requested, int form) {
+constexpr double kSegSeedNs = 6.0;
+WalkChoice pick_band_k_f64(const mp_ctx* c, int W, int requested, int form, bool seg_ok) {
     const int64_t S = 4LL * std::max(1, c->cus);
+    if (form == 4) return WalkChoice{requested > 0 ? requested : 4, 4};
     auto items_of = [&](int k) { return (static_cast<int64_t>(mp::walk_f64_nbands(W, k)) + 1) / 2; };
     auto form_of = [&](int k) {
         if (k != 8) return 0;
@@ -151,7 +158,22 @@ WalkChoice pick_band_k_f64(const mp_ctx* c, int W, int requested, int form) {
             best = WalkChoice{k, form_of(k)};
         }
     }
+    if (seg_ok && form == 0 && W <= mp::kSegMaxW) {
+        const double cells = 0.5 * static_cast<double>(W) * (static_cast<double>(W) + 1.0);
+        const double t_seg =
+            cells * (0.75 * 3.0 * 0.57) / (3.0 * static_cast<double>(S)) + kSegSeedNs * static_cast<double>(W);
+        if (t_seg < best_t * static_cast<double>(W)) return WalkChoice{4, 4};
+    }
     return best;
+}
+
+// The walker choice of a self-join and the scratch bytes it needs.
+static WalkChoice exact_choice(const mp_ctx* c, const Geometry& g, const mp_opts* o, bool seg_ok) {
+    return pick_band_k_f64(c, g.W, o ? o->band_k : 0, o ? o->walk_form : 0, seg_ok);
+}
+static size_t exact_need(const Geometry& g, const WalkChoice& wc) {
+    const bool seg = wc.form == 4 && (wc.band_k == 1 || wc.band_k == 2 || wc.band_k == 4);
+    return mp::exact_scratch_bytes(g.n, g.P, g.W) + (seg ? mp::seg_ckpt_bytes(g.W, wc.band_k) : 0);
 }
 
 // ord(-1.0) in the order-preserving uint64 map used by the exact path's keys
@@ -278,19 +300,21 @@ int mp_memcpy_d2h(mp_ctx* c, void* dst, const void* src, uint64_t bytes) {
 // ---------------------------------------------------------------------------
 // exact FP64 self-join
 // ---------------------------------------------------------------------------
-// The exact self-join on `scratch` (at least mp::exact_scratch_bytes(n, P, W)
-// bytes of device memory); every buffer is written before it is read.
+// The exact self-join on `scratch` (at least exact_need(g, wc) bytes of device
+// memory); every buffer is written before it is read.
 static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const Geometry& g,
-                            double* d_profile, int64_t* d_index, const mp_opts* o, mp_timing* tm) {
+                            double* d_profile, int64_t* d_index, const mp_opts* o, mp_timing* tm,
+                            const WalkChoice& wc) {
     hipStream_t st = c->stream;
     const int64_t n = g.n, P = g.P;
     const int m = g.m;
     const int64_t index_init = (o && o->index_init == 0) ? 0 : -1;
     const int W = g.W;
-    if (o && (o->walk_form < 0 || o->walk_form > 3)) return MP_ERR_BAD_ARG;
-    const WalkChoice wc = pick_band_k_f64(c, W, o ? o->band_k : 0, o ? o->walk_form : 0);
+    if (o && (o->walk_form < 0 || o->walk_form > 4)) return MP_ERR_BAD_ARG;
     const int band_k = wc.band_k;
     if (band_k != 1 && band_k != 2 && band_k != 4 && band_k != 8) return MP_ERR_BAD_ARG;
+    const bool seg = wc.form == 4;
+    if (seg && band_k == 8) return MP_ERR_BAD_ARG;
     Carve cv{scratch};
     double* A = cv.take<double>(n + 64);
     double* cs = cv.take<double>(n);
@@ -302,6 +326,7 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
     auto* val = cv.take<unsigned long long>(P);
     auto* key = cv.take<unsigned long long>(P);
     auto* counter = cv.take<unsigned int>(1);
+    double* ckpt = seg ? cv.take<double>(mp::walk_f64_seg_items(W, band_k) * 64 * band_k) : nullptr;
     // key codes: partners (band width 1), or a partner and a direction bit
     const int ib = band_k > 1 ? bits_for(2 * P + 2) : bits_for(P + 1);
 
@@ -340,6 +365,11 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
         const int waves = (nb + 1) / 2;  // at most one wave per two bands
         a.form = wc.form;
         a.groups = std::max(1, std::min((waves + 3) / 4, ((a.form == 2 || a.form == 3) ? 2 : 8) * c->cus));
+        if (seg) {  // one wave per segment, up to three groups per CU
+            a.ckpt = ckpt;
+            a.groups = static_cast<int>(
+                std::max<int64_t>(1, std::min<int64_t>((mp::walk_f64_seg_items(W, band_k) + 3) / 4, 3LL * c->cus)));
+        }
         MP_TRY(mp::launch_walk_f64(a, st));
     }
     MP_HIP(hipEventRecord(c->ev[3], st));
@@ -355,19 +385,21 @@ static int selfjoin_f64_run(mp_ctx* c, char* scratch, const double* d_ts, const 
         tm->finalize_ms = elapsed(c->ev[3], c->ev[4]);
         tm->total_ms = elapsed(c->ev[0], c->ev[4]);
         tm->cells = g.cells;
-        tm->work_items = W > 0 ? mp::walk_f64_nbands(W, band_k) : 0;
+        tm->work_items = W > 0 ? (seg ? mp::walk_f64_seg_items(W, band_k) : mp::walk_f64_nbands(W, band_k)) : 0;
         tm->band_k = band_k;
         tm->tile_len = 0;
-        tm->walk_form = (band_k == 8) ? wc.form : 0;
+        tm->walk_form = (band_k == 8 || seg) ? wc.form : 0;
     }
     return MP_OK;
 }
 
 static int selfjoin_f64_locked(mp_ctx* c, const double* d_ts, const Geometry& g, double* d_profile,
                                int64_t* d_index, const mp_opts* o, mp_timing* tm) {
-    MP_TRY(c->work.reserve(mp::exact_scratch_bytes(g.n, g.P, g.W)));
+    const WalkChoice wc = exact_choice(c, g, o, true);
+    if (wc.form == 4 && g.W > mp::kSegMaxW) return MP_ERR_BAD_ARG;  // segments: short series only
+    MP_TRY(c->work.reserve(exact_need(g, wc)));
     c->work.used = 0;
-    return selfjoin_f64_run(c, c->work.base, d_ts, g, d_profile, d_index, o, tm);
+    return selfjoin_f64_run(c, c->work.base, d_ts, g, d_profile, d_index, o, tm, wc);
 }
 
 int mp_selfjoin_f64_dev(mp_ctx* c, const double* d_ts, int64_t n, int32_t m, double* d_profile,
@@ -389,7 +421,12 @@ int mp_selfjoin_f64_dev_noalloc(mp_ctx* c, const double* d_ts, int64_t n, int32_
     Geometry g;
     MP_TRY(geometry(n, m, o, &g));
     if (n > mp::kNoAllocMaxPoints) return MP_ERR_TOO_LONG;
-    const size_t need = mp::exact_scratch_bytes(g.n, g.P, g.W);
+    // column segments where their start values fit the array too, else whole bands
+    WalkChoice wc = exact_choice(c, g, o, true);
+    if (wc.form == 4 && g.W > mp::kSegMaxW) return MP_ERR_BAD_ARG;
+    if (wc.form == 4 && exact_need(g, wc) > mp::kNoAllocScratchBytes && !(o && o->walk_form == 4))
+        wc = exact_choice(c, g, o, false);
+    const size_t need = exact_need(g, wc);
     if (need > mp::kNoAllocScratchBytes) return MP_ERR_TOO_LONG;
     char* scratch = nullptr;
     std::mutex* use = nullptr;
@@ -397,7 +434,7 @@ int mp_selfjoin_f64_dev_noalloc(mp_ctx* c, const double* d_ts, int64_t n, int32_
     std::lock_guard<std::mutex> guard(c->lock);
     std::lock_guard<std::mutex> guard2(*use);
     MP_HIP(hipSetDevice(c->device));
-    return selfjoin_f64_run(c, scratch, d_ts, g, d_profile, d_index, o, tm);
+    return selfjoin_f64_run(c, scratch, d_ts, g, d_profile, d_index, o, tm, wc);
 }
 
 // Host-pointer wrapper shared by the self-joins and PreSCRIMP.  The default

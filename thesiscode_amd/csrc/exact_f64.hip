@@ -424,12 +424,17 @@ __device__ __forceinline__ double max_tree(const double* sc) {
 // 32 s + 16 (s / 8)): the 16 lanes of an LDS cycle then cover all 64 banks.
 // FEEDLDS (rows by DPP only): the row entering lane 63 comes from LDS staging
 // (below) instead of from a scalar load and eight moves into vector registers.
-template <int K, bool WRITEBACK, bool RING, bool FEEDLDS>
+//
+// SEGS: the items are column segments of the bands (self-join only, seg_r =
+// SEG / BW, start values in ckpt); a segment that ends before its band does
+// flushes the rows still in the band, whose other cells the next segment walks.
+template <int K, bool WRITEBACK, bool RING, bool FEEDLDS, bool SEGS = false>
 __device__ __forceinline__ void walk_body(
     const WalkRec* __restrict__ colrec, const WalkRec* __restrict__ rowrec,
     double* qt, unsigned long long* rowval, unsigned long long* rowkey,
     unsigned long long* colval, unsigned long long* colkey, unsigned int* counter,
-    int64_t row_off, int64_t col_off, int blocklen, int tl, int m, int ib, int nbands) {
+    int64_t row_off, int64_t col_off, int blocklen, int tl, int m, int ib, int nbands,
+    int seg_r = 0, const double* __restrict__ ckpt = nullptr) {
     // the ring form (one wave per SIMD, latency-bound) flushes every 32 steps,
     // the others every 16
     constexpr int F = RING ? 2 * kFlush : kFlush;
@@ -439,6 +444,7 @@ __device__ __forceinline__ void walk_body(
     constexpr int BW = kWave * K;
     static_assert(F % K == 0, "flush period must be a multiple of K");
     static_assert(!(RING && FEEDLDS), "the ring form reads every row from LDS");
+    static_assert(!SEGS || (!RING && !WRITEBACK), "segments: DPP rows, self-join only");
     // Column partials, one row of 64 per step, padded by one entry: a flush
     // lane reads PQ consecutive entries of one row (immediate offsets), and
     // the lanes of an LDS phase then fall on distinct banks.  Without the
@@ -576,14 +582,22 @@ __device__ __forceinline__ void walk_body(
         __builtin_amdgcn_wave_barrier();
     };
 
+    const int nitems = SEGS ? static_cast<int>(seg_items(nbands, seg_r)) : nbands;
     for (;;) {
         // One band per item, longest first (band 0 has the most columns):
         // waves that finish early take the remaining, shorter bands, so all
-        // waves end within about one short band of each other.
-        const int band = next_item(counter, lane);
-        if (band >= nbands) break;
+        // waves end within about one short band of each other.  SEGS: one
+        // segment per item, band by band; columns [cb, L) of the band.
+        const int item = next_item(counter, lane);
+        if (item >= nitems) break;
+        int band = item, cb = 0;
+        if constexpr (SEGS) {
+            band = seg_band_of(item, nbands, seg_r);
+            cb = static_cast<int>(item - seg_first_item(band, nbands, seg_r)) * seg_r * BW;
+        }
         const int dbase = band * BW;
-        const int L = tl - dbase;
+        const int Lb = tl - dbase;  // the band's columns
+        const int L = SEGS ? min(cb + seg_r * BW, Lb) : Lb;
         // Cells outside the triangle are masked by NaN: a row past the last
         // one is a padding record with sInv = NaN, a diagonal past the block
         // gets QT = NaN, and a NaN score never wins a comparison (nor does
@@ -596,8 +610,13 @@ __device__ __forceinline__ void walk_body(
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int d = dbase + lid0 + k;
-            q[k] = (d < blocklen) ? qt[d] : QNAN;
-            const WalkRec r = rowrec[d];
+            if constexpr (SEGS) {
+                q[k] = (d < blocklen) ? ((cb == 0) ? qt[d] : ckpt[static_cast<int64_t>(item) * BW + lid0 + k])
+                                      : QNAN;
+            } else {
+                q[k] = (d < blocklen) ? qt[d] : QNAN;
+            }
+            const WalkRec r = rowrec[d + cb];
             wa[k] = r.a;
             wam[k] = r.am;
             ws[k] = r.s;
@@ -816,22 +835,22 @@ __device__ __forceinline__ void walk_body(
             pc = colval[jj];
             pr = rowval[jj + dbase];
         };
-        int c0 = 0;
+        int c0 = cb;  // a multiple of 2F: the first window reads feed buffer 0
         uint64_t pc = 0, pr = 0;
-        if constexpr (!RING) current(0, pc, pr);
+        if constexpr (!RING) current(cb, pc, pr);
         if constexpr (FEEDLDS) {
-            // the first window's feed rows (band rows BW + x) to buffer 0
+            // the first window's feed rows (band rows cb + BW + x) to buffer 0
             __builtin_amdgcn_wave_barrier();
             if (lane < 2 * F) {
                 const char* src =
-                    reinterpret_cast<const char*>(rowrec + (dbase + BW + (lane >> 1))) + 16 * (lane & 1);
+                    reinterpret_cast<const char*>(rowrec + (dbase + cb + BW + (lane >> 1))) + 16 * (lane & 1);
                 __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src),
                                                  (__attribute__((address_space(3))) void*)(feed), 16, 0, 0);
             }
             __builtin_amdgcn_wave_barrier();
         }
-        cur_c = colrec[0];
-        if constexpr (!RING && !FEEDLDS) cur_f = rowrec[dbase + BW];
+        cur_c = colrec[cb];
+        if constexpr (!RING && !FEEDLDS) cur_f = rowrec[dbase + cb + BW];
         // let the band's set-up loads land here, so the windows never wait on
         // vector memory (whose counter would also cover the flush's atomics)
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
@@ -855,6 +874,35 @@ __device__ __forceinline__ void walk_body(
             window(c0, std::true_type{});
             if constexpr (RING || FEEDLDS) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             flush(c0, L - c0, dbase, pc, pr, [](uint64_t&, uint64_t&) {});
+        }
+        if constexpr (SEGS) {
+            // A segment that ends before its band (L a multiple of K): slot k
+            // of each lane holds row L + dbase + lid0 + k.  Slot K-1's row has
+            // just entered the lane (no cell here yet: only rin counts); the
+            // others' lm holds this lane's cells of them so far.  Their
+            // maxima so far are flushed as the rows leaving lane 0 are (the
+            // code names the block of the lane that holds the maximum).
+            if (L < Lb) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const bool own = (k < K - 1) && (lm[k] > rin[k]);
+                    const double v = (k < K - 1) ? dmax(rin[k], lm[k]) : rin[k];
+                    const int t = own ? lane + 1 : tag[k];
+                    const int jj = L + lid0 + k;  // the column at which the row leaves lane 0
+                    const int i = jj + dbase;
+                    if (t > 0 && i < tl && v == v) {
+                        const uint64_t orw = ord_f64(v);
+                        int64_t rcode;
+                        if constexpr (K == 1) {
+                            rcode = static_cast<int64_t>(jj) - (t - 1) + row_off;
+                        } else {
+                            rcode = ((static_cast<int64_t>(jj) - (t - 1) * K + row_off) << 1) | 1;
+                        }
+                        atomicMax(&rowval[i], static_cast<unsigned long long>(orw));
+                        atomicMax(&rowkey[i], static_cast<unsigned long long>(make_key(orw, ib, rcode)));
+                    }
+                }
+            }
         }
     }
 }
@@ -1201,6 +1249,61 @@ __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(3))) voi
     MP_WALK_PARAMS) {
     walk_body<K, WRITEBACK, false, true>(MP_WALK_ARGS);
 }
+// Start values of the column segments: diagonal d of band b advanced from its
+// seed at column 0 to each segment start s*SEG (s = 1 .. n_b - 1) by the
+// walker's own recurrence, the same two FMAs in the same order, so that a
+// segment continues exactly as the whole band's walk would.  One lane per
+// diagonal; each group of G steps' row and column records is loaded while the
+// chain of the group before runs.
+__global__ __launch_bounds__(256) void k_seg_seeds(const WalkRec* __restrict__ colrec,
+                                                   const WalkRec* __restrict__ rowrec,
+                                                   const double* __restrict__ qt, double* __restrict__ ckpt,
+                                                   int tl, int blocklen, int bw, int nbands, int seg_r) {
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    const int b = d / bw;
+    if (b >= nbands || d >= blocklen) return;
+    const int seg = seg_r * bw;
+    const int nseg = (tl - b * bw + seg - 1) / seg;
+    if (nseg <= 1) return;
+    double* const out = ckpt + seg_first_item(b, nbands, seg_r) * bw + (d - b * bw);
+    // (a, am) of row d + j and of column j
+    const double2* const rp = reinterpret_cast<const double2*>(rowrec + d);
+    const double2* const cp = reinterpret_cast<const double2*>(colrec);
+    constexpr int G = 16;
+    const int steps = (nseg - 1) * seg;  // a multiple of 2G
+    double q = qt[d];
+    double2 ra[G], ca[G], rb[G], cb[G];
+    auto load = [&](int j, double2(&r)[G], double2(&c)[G]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            r[u] = rp[2 * (j + u)];
+            c[u] = cp[2 * (j + u)];
+        }
+    };
+    auto run = [&](const double2(&r)[G], const double2(&c)[G]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            q = fma(r[u].y, c[u].y, q);   // + am(row) * am(column)
+            q = fma(-r[u].x, c[u].x, q);  // - a(row) * a(column)
+        }
+    };
+    load(0, ra, ca);
+    for (int j = 0; j < steps; j += 2 * G) {
+        load(j + G, rb, cb);
+        run(ra, ca);
+        if ((j + G) % seg == 0) out[static_cast<int64_t>((j + G) / seg) * bw] = q;
+        if (j + 2 * G < steps) load(j + 2 * G, ra, ca);
+        run(rb, cb);
+        if ((j + 2 * G) % seg == 0) out[static_cast<int64_t>((j + 2 * G) / seg) * bw] = q;
+    }
+}
+
+// Column segments of k_walk_f64's bands (see walk_body); seg_r = SEG / BW.
+template <int K>
+__global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_f64_seg(
+    MP_WALK_PARAMS, int seg_r, const double* __restrict__ ckpt) {
+    walk_body<K, false, false, true, true>(MP_WALK_ARGS, seg_r, ckpt);
+}
 // K = 8 holds twice the row records (one wave per SIMD); its rows come from the
 // LDS ring, which the one group per CU this leaves has room for.
 template <int K, bool WRITEBACK>
@@ -1364,9 +1467,37 @@ int walk_f64_nbands(int blocklen, int band_k) {
     return (blocklen + bw - 1) / bw;
 }
 
+int64_t walk_f64_seg_items(int blocklen, int band_k) {
+    return seg_items(walk_f64_nbands(blocklen, band_k), kSegCols / (kWave * band_k));
+}
+
 int launch_walk_f64(const WalkF64Launch& a, hipStream_t st) {
     const int nbands = walk_f64_nbands(a.blocklen, a.band_k);
     if (nbands <= 0) return MP_OK;
+    if (a.form == 4) {
+        // the self-join only: one block of every diagonal, no write-back
+        if (a.writeback || a.blocklen != a.tl || !a.ckpt) return MP_ERR_BAD_ARG;
+        const int bw = kWave * a.band_k, seg_r = kSegCols / bw;
+        const int64_t threads = static_cast<int64_t>(nbands) * bw;
+        hipLaunchKernelGGL(k_seg_seeds, dim3(static_cast<unsigned>((threads + 255) / 256)), dim3(256), 0, st,
+                           a.colrec, a.rowrec, a.qt, a.ckpt, a.tl, a.blocklen, bw, nbands, seg_r);
+        if (hipGetLastError() != hipSuccess) return MP_ERR_DEVICE;
+#define MP_SEG_CASE(KK)                                                                                   \
+    case KK:                                                                                                \
+        hipLaunchKernelGGL((k_walk_f64_seg<KK>), dim3(a.groups), dim3(kGroup), 0, st, a.colrec, a.rowrec, \
+                           a.qt, a.rowval, a.rowkey, a.colval, a.colkey, a.counter, a.row_off, a.col_off,   \
+                           a.blocklen, a.tl, a.m, a.ib, nbands, seg_r, static_cast<const double*>(a.ckpt)); \
+        break;
+        switch (a.band_k) {
+            MP_SEG_CASE(1)
+            MP_SEG_CASE(2)
+            MP_SEG_CASE(4)
+            default:
+                return MP_ERR_BAD_ARG;
+        }
+#undef MP_SEG_CASE
+        return hipGetLastError() == hipSuccess ? MP_OK : MP_ERR_DEVICE;
+    }
     const dim3 grid(a.groups), block(kGroup);
 #define MP_WALK_CASE(KK, KERNEL)                                                               \
     case KK:                                                                                     \

@@ -66,6 +66,47 @@ int launch_unpack_block(const unsigned long long* val, const unsigned long long*
 int launch_pack_rec(const double* A, const double* s, const double* mu, int m, int valid,
                     int total, WalkRec* out, hipStream_t st);
 
+// Column segments.  On short series the bands are too few to occupy the GPU and
+// the walk lasts as long as its longest band.  The DPP-row walker (K <= 4) can
+// then cut band b (L_b = tl - b*BW columns) at every multiple of SEG = r*BW
+// columns, into n_b = ceil(L_b / SEG) = ceil((nbands - b) / r) segments, and
+// hand them out band by band: item i is segment s of the band b with
+// seg_first_item(b) <= i < seg_first_item(b + 1), s = i - seg_first_item(b).
+// A segment s > 0 starts from its diagonals' running dot products at column
+// s*SEG, which k_seg_seeds computes beforehand by the walker's own recurrence
+// and stores at ckpt[i * BW + (d - b*BW)].  A row whose cells a segment
+// boundary splits gets one partial result per segment, merged by the result
+// keys as the parts of a row that two bands cover are.
+constexpr int kSegCols = 2048;                    // SEG, a multiple of every BW <= 256
+constexpr int64_t kSegMaxW = int64_t(1) << 18;    // longest walk cut into segments
+// sum over u = 1 .. U of ceil(u / r)
+__host__ __device__ constexpr int64_t seg_ceil_sum(int64_t U, int64_t r) {
+    const int64_t q = U / r, t = U % r;
+    return r * q * (q + 1) / 2 + t * (q + 1);
+}
+__host__ __device__ constexpr int64_t seg_first_item(int64_t b, int64_t nbands, int64_t r) {
+    return seg_ceil_sum(nbands, r) - seg_ceil_sum(nbands - b, r);
+}
+__host__ __device__ constexpr int64_t seg_items(int64_t nbands, int64_t r) { return seg_ceil_sum(nbands, r); }
+// the band of item i: the largest b with seg_first_item(b) <= i
+__host__ __device__ inline int seg_band_of(int64_t i, int nbands, int r) {
+    int lo = 0, hi = nbands - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (seg_first_item(mid, nbands, r) <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+// device bytes of the segment start values for W diagonals at band width K
+constexpr size_t seg_ckpt_bytes(int64_t W, int band_k) {
+    const int64_t bw = 64 * band_k;
+    const int64_t nb = (W + bw - 1) / bw;
+    return exact_aligned(seg_items(nb, kSegCols / bw) * bw, 8);
+}
+
 struct WalkF64Launch {
     const WalkRec* colrec;  // column j: colrec[j], j < tl
     const WalkRec* rowrec;  // row r: rowrec[r], r < tl + kWalkRecPad (padding past tl)
@@ -77,8 +118,12 @@ struct WalkF64Launch {
     int form;               // band_k = 8: 1 rows from an LDS ring, one wave per SIMD
                             // (k_walk_f64_wide); 2 rows by DPP, two waves per SIMD
                             // (k_walk_f64_pair); 3 rows in place, diagonals rotating
-                            // by DPP, two waves per SIMD (k_walk_f64_rot, no write-back)
+                            // by DPP, two waves per SIMD (k_walk_f64_rot, no write-back);
+                            // 4 (band_k <= 4) column segments (k_walk_f64_seg, no write-back)
+    double* ckpt;           // form 4: segment start values (seg_ckpt_bytes)
 };
+// form 4: the number of segments (work items) of the launch
+int64_t walk_f64_seg_items(int blocklen, int band_k);
 int walk_f64_nbands(int blocklen, int band_k);
 int launch_walk_f64(const WalkF64Launch& a, hipStream_t st);
 
