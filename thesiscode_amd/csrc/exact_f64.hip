@@ -1253,48 +1253,87 @@ __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(3))) voi
 // seed at column 0 to each segment start s*SEG (s = 1 .. n_b - 1) by the
 // walker's own recurrence, the same two FMAs in the same order, so that a
 // segment continues exactly as the whole band's walk would.  One lane per
-// diagonal; each group of G steps' row and column records is loaded while the
-// chain of the group before runs.
+// diagonal, one wave per 64 diagonals of a band.  As in the walker, lane l
+// holds row j + d at column j and the rows shift down one lane per step (DPP),
+// lane 63 taking row j + d0 + 64; the column records and those entering rows
+// of each 64-step tile are staged in LDS one tile ahead and read by broadcast.
 __global__ __launch_bounds__(256) void k_seg_seeds(const WalkRec* __restrict__ colrec,
                                                    const WalkRec* __restrict__ rowrec,
                                                    const double* __restrict__ qt, double* __restrict__ ckpt,
                                                    int tl, int blocklen, int bw, int nbands, int seg_r) {
-    const int d = blockIdx.x * 256 + threadIdx.x;
-    const int b = d / bw;
-    if (b >= nbands || d >= blocklen) return;
+    constexpr int T = kWave;  // steps per tile
+    __shared__ double2 s_col[4][2][T], s_fill[4][2][T];
+    const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int d0 = (blockIdx.x * 4 + wv) * kWave;
+    const int b = d0 / bw;
+    if (b >= nbands || d0 >= blocklen) return;
     const int seg = seg_r * bw;
     const int nseg = (tl - b * bw + seg - 1) / seg;
     if (nseg <= 1) return;
+    const int d = d0 + lane;
     double* const out = ckpt + seg_first_item(b, nbands, seg_r) * bw + (d - b * bw);
-    // (a, am) of row d + j and of column j
-    const double2* const rp = reinterpret_cast<const double2*>(rowrec + d);
-    const double2* const cp = reinterpret_cast<const double2*>(colrec);
-    constexpr int G = 16;
-    const int steps = (nseg - 1) * seg;  // a multiple of 2G
-    double q = qt[d];
-    double2 ra[G], ca[G], rb[G], cb[G];
-    auto load = [&](int j, double2(&r)[G], double2(&c)[G]) __attribute__((always_inline)) {
-#pragma unroll
-        for (int u = 0; u < G; ++u) {
-            r[u] = rp[2 * (j + u)];
-            c[u] = cp[2 * (j + u)];
-        }
+    const int steps = (nseg - 1) * seg;  // a multiple of T
+    // (a, am) of record r: the first 16 bytes
+    auto ld = [](const WalkRec* r) __attribute__((always_inline)) {
+        return *reinterpret_cast<const double2*>(r);
     };
-    auto run = [&](const double2(&r)[G], const double2(&c)[G]) __attribute__((always_inline)) {
-#pragma unroll
-        for (int u = 0; u < G; ++u) {
-            q = fma(r[u].y, c[u].y, q);   // + am(row) * am(column)
-            q = fma(-r[u].x, c[u].x, q);  // - a(row) * a(column)
+    double q = (d < blocklen) ? qt[d] : 0.0;
+    const double2 r0 = ld(rowrec + d);
+    double ra = r0.x, ram = r0.y;
+    // tile 0 to buffer 0
+    s_col[wv][0][lane] = ld(colrec + lane);
+    s_fill[wv][0][lane] = ld(rowrec + d0 + kWave + lane);
+    __builtin_amdgcn_wave_barrier();
+    for (int j0 = 0; j0 < steps; j0 += T) {
+        const int buf = (j0 / T) & 1;
+        // the next tile's records, written to the other buffer after this tile
+        double2 nc = make_double2(0.0, 0.0), nf = make_double2(0.0, 0.0);
+        const bool more = j0 + T < steps;
+        if (more) {
+            nc = ld(colrec + j0 + T + lane);
+            nf = ld(rowrec + j0 + T + d0 + kWave + lane);
         }
-    };
-    load(0, ra, ca);
-    for (int j = 0; j < steps; j += 2 * G) {
-        load(j + G, rb, cb);
-        run(ra, ca);
-        if ((j + G) % seg == 0) out[static_cast<int64_t>((j + G) / seg) * bw] = q;
-        if (j + 2 * G < steps) load(j + 2 * G, ra, ca);
-        run(rb, cb);
-        if ((j + 2 * G) % seg == 0) out[static_cast<int64_t>((j + 2 * G) / seg) * bw] = q;
+        const double2* const cb = s_col[wv][buf];
+        const double2* const fb = s_fill[wv][buf];
+        // U steps' records are read from LDS one group of U steps ahead (the
+        // scheduling barriers keep the reads there), so no step waits for them
+        constexpr int U = 8;
+        double2 cA[U], fA[U], cB[U], fB[U];
+        auto rd = [&](int t0, double2(&c)[U], double2(&f)[U]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c[u] = cb[t0 + u];
+                f[u] = fb[t0 + u];
+            }
+        };
+        auto run = [&](const double2(&c)[U], const double2(&f)[U]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                q = fma(ram, c[u].y, q);  // + am(row) * am(column)
+                q = fma(-ra, c[u].x, q);  // - a(row) * a(column)
+                ra = shift_down_f64(ra, f[u].x);
+                ram = shift_down_f64(ram, f[u].y);
+            }
+        };
+        rd(0, cA, fA);
+#pragma unroll
+        for (int t0 = 0; t0 < T; t0 += 2 * U) {
+            rd(t0 + U, cB, fB);
+            __builtin_amdgcn_sched_barrier(0);
+            run(cA, fA);
+            __builtin_amdgcn_sched_barrier(0);
+            if (t0 + 2 * U < T) rd(t0 + 2 * U, cA, fA);
+            __builtin_amdgcn_sched_barrier(0);
+            run(cB, fB);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if ((j0 + T) % seg == 0 && d < blocklen) out[static_cast<int64_t>((j0 + T) / seg) * bw] = q;
+        __builtin_amdgcn_wave_barrier();
+        if (more) {
+            s_col[wv][buf ^ 1][lane] = nc;
+            s_fill[wv][buf ^ 1][lane] = nf;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
