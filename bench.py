@@ -117,6 +117,8 @@ def kernel_name(precision: str, tm: dict, accum64: bool = True) -> str:
     k = tm["band_k"]
     if precision == "f32r":
         return f"k_tiles_f32<{k},{'true' if accum64 else 'false'}>" if k <= 8 else f"k_tiles_f32_wide<{k}>"
+    if tm.get("walk_form", 0) == 4:  # column segments: the start-value pre-pass, then the walk
+        return f"k_seg_seeds + k_walk_f64_seg<{k}>"
     form = {1: "k_walk_f64_wide", 2: "k_walk_f64_pair", 3: "k_walk_f64_rot"}.get(tm.get("walk_form", 0))
     return f"{form}<{k}>" if (k == 8 and form) else f"k_walk_f64<{k}>"
 

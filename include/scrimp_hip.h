@@ -56,12 +56,16 @@ typedef struct mp_opts {
                             every 16 columns; within 1e-4 of the FP64 profile. 0 = FP32
                             accumulation, about 10% faster, drifts to ~1e-3 on long
                             near-periodic series                                          */
-    int32_t walk_form;   /* exact path, band_k = 8: how the band walker runs.  0 = chosen
-                            from the work size; 1 = one wave per SIMD, rows from an LDS
-                            ring; 2 = two waves per SIMD, rows shifted by DPP; 3 = two
-                            waves per SIMD, rows in place and the diagonals' running
-                            dot products rotated by DPP.  All give the same profile
-                            values                                                      */
+    int32_t walk_form;   /* exact path: how the band walker runs.  0 = chosen from the
+                            work size; at band_k = 8: 1 = one wave per SIMD, rows from
+                            an LDS ring; 2 = two waves per SIMD, rows shifted by DPP;
+                            3 = two waves per SIMD, rows in place and the diagonals'
+                            running dot products rotated by DPP; at band_k = 1, 2 or 4
+                            (default 4): 4 = every band cut into 2048-column segments,
+                            one wave each, started from values a pre-pass advances
+                            along the diagonals (series with W <= 2^18 diagonals; the
+                            automatic choice on short series).  All give the same
+                            profile values                                              */
     int32_t reserved;
 } mp_opts;
 
@@ -74,10 +78,11 @@ typedef struct mp_timing {
     double kernel_ms;    /* diagonal band / tile kernel                                    */
     double finalize_ms;  /* refine + unpack to profile/index                               */
     double cells;        /* cells evaluated: W(W+1)/2, W = P - excl - 1                     */
-    int64_t work_items;  /* bands (exact path) or tiles (FP32 path) processed              */
+    int64_t work_items;  /* bands or column segments (exact path), tiles (FP32 path)      */
     int32_t band_k, tile_len;
-    int32_t walk_form;   /* exact path at band width 8: the walker form that ran (mp_opts.walk_form
-                            numbering: 1 LDS ring, 2 DPP rows, 3 rotating); 0 otherwise */
+    int32_t walk_form;   /* exact path: the walker form that ran (mp_opts.walk_form
+                            numbering: 1 LDS ring, 2 DPP rows, 3 rotating, 4 column
+                            segments); 0 for whole bands at band width 1, 2 or 4       */
     int32_t reserved;
 } mp_timing;
 
@@ -116,7 +121,10 @@ int mp_selfjoin_f64_dev(mp_ctx* ctx, const double* d_ts, int64_t n, int32_t m,
  * growable arena, so the call allocates and frees nothing.  The array holds
  * series of up to 524,288 points (2^19, about 50 MB on every GPU the library is
  * loaded on, kept until the process ends); a longer series returns
- * MP_ERR_TOO_LONG.  All calls on one GPU share the array: they are serialised
+ * MP_ERR_TOO_LONG.  Column segments (walk_form 4) run where their start values
+ * fit the array as well; otherwise the automatic choice walks whole bands (the
+ * same results) and a forced walk_form 4 returns MP_ERR_TOO_LONG.  All calls on
+ * one GPU share the array: they are serialised
  * inside the library (a lock per GPU), and a call on another stream or process
  * that overlaps one on the same GPU without this library's lock would race.
  * The array keeps the last call's contents; every buffer in it is written
