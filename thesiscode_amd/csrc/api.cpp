@@ -128,7 +128,7 @@ struct WalkChoice {
     int band_k;
     int form;  // K = 8: the launch's form (1 LDS ring, 2 DPP rows at two waves per SIMD, 3 rotating diagonals)
 };
-constexpr double kSegSeedNs = 21.0;  // measured: 1.35 ms over the 63,488 columns of n = 65,536
+constexpr double kSegSeedNs = 18.0;  // measured: 1.13 ms over the 63,488 columns of n = 65,536
 WalkChoice pick_band_k_f64(const mp_ctx* c, int W, int requested, int form, bool seg_ok) {
     const int64_t S = 4LL * std::max(1, c->cus);
     if (form == 4) return WalkChoice{requested > 0 ? requested : 4, 4};

@@ -1253,16 +1253,17 @@ __global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(3))) voi
 // seed at column 0 to each segment start s*SEG (s = 1 .. n_b - 1) by the
 // walker's own recurrence, the same two FMAs in the same order, so that a
 // segment continues exactly as the whole band's walk would.  One lane per
-// diagonal, one wave per 64 diagonals of a band.  As in the walker, lane l
-// holds row j + d at column j and the rows shift down one lane per step (DPP),
-// lane 63 taking row j + d0 + 64; the column records and those entering rows
-// of each 64-step tile are staged in LDS one tile ahead and read by broadcast.
+// diagonal, one wave per 64 diagonals d0 .. d0+63 of a band.  Each 64-step tile
+// (columns j0 .. j0+63) stages its column records and the rows it meets, j0 + d0
+// .. j0 + d0 + 127, in LDS one tile ahead; at step t lane l reads row t + l of
+// the tile (consecutive lanes, consecutive entries) and the column by
+// broadcast, one group of 8 steps ahead, so a step is its two FMAs.
 __global__ __launch_bounds__(256) void k_seg_seeds(const WalkRec* __restrict__ colrec,
                                                    const WalkRec* __restrict__ rowrec,
                                                    const double* __restrict__ qt, double* __restrict__ ckpt,
                                                    int tl, int blocklen, int bw, int nbands, int seg_r) {
     constexpr int T = kWave;  // steps per tile
-    __shared__ double2 s_col[4][2][T], s_fill[4][2][T];
+    __shared__ double2 s_row[4][2][2 * T], s_col[4][2][T];
     const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int d0 = (blockIdx.x * 4 + wv) * kWave;
     const int b = d0 / bw;
@@ -1278,60 +1279,57 @@ __global__ __launch_bounds__(256) void k_seg_seeds(const WalkRec* __restrict__ c
         return *reinterpret_cast<const double2*>(r);
     };
     double q = (d < blocklen) ? qt[d] : 0.0;
-    const double2 r0 = ld(rowrec + d);
-    double ra = r0.x, ram = r0.y;
     // tile 0 to buffer 0
+    s_row[wv][0][lane] = ld(rowrec + d0 + lane);
+    s_row[wv][0][T + lane] = ld(rowrec + d0 + T + lane);
     s_col[wv][0][lane] = ld(colrec + lane);
-    s_fill[wv][0][lane] = ld(rowrec + d0 + kWave + lane);
     __builtin_amdgcn_wave_barrier();
     for (int j0 = 0; j0 < steps; j0 += T) {
         const int buf = (j0 / T) & 1;
         // the next tile's records, written to the other buffer after this tile
-        double2 nc = make_double2(0.0, 0.0), nf = make_double2(0.0, 0.0);
+        double2 n0 = make_double2(0.0, 0.0), n1 = n0, nc = n0;
         const bool more = j0 + T < steps;
         if (more) {
+            n0 = ld(rowrec + j0 + T + d0 + lane);
+            n1 = ld(rowrec + j0 + 2 * T + d0 + lane);
             nc = ld(colrec + j0 + T + lane);
-            nf = ld(rowrec + j0 + T + d0 + kWave + lane);
         }
+        const double2* const rb = &s_row[wv][buf][lane];
         const double2* const cb = s_col[wv][buf];
-        const double2* const fb = s_fill[wv][buf];
-        // U steps' records are read from LDS one group of U steps ahead (the
-        // scheduling barriers keep the reads there), so no step waits for them
         constexpr int U = 8;
-        double2 cA[U], fA[U], cB[U], fB[U];
-        auto rd = [&](int t0, double2(&c)[U], double2(&f)[U]) __attribute__((always_inline)) {
+        double2 rA[U], cA[U], rB[U], cB[U];
+        auto rd = [&](int t0, double2(&r)[U], double2(&c)[U]) __attribute__((always_inline)) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
+                r[u] = rb[t0 + u];
                 c[u] = cb[t0 + u];
-                f[u] = fb[t0 + u];
             }
         };
-        auto run = [&](const double2(&c)[U], const double2(&f)[U]) __attribute__((always_inline)) {
+        auto run = [&](const double2(&r)[U], const double2(&c)[U]) __attribute__((always_inline)) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                q = fma(ram, c[u].y, q);  // + am(row) * am(column)
-                q = fma(-ra, c[u].x, q);  // - a(row) * a(column)
-                ra = shift_down_f64(ra, f[u].x);
-                ram = shift_down_f64(ram, f[u].y);
+                q = fma(r[u].y, c[u].y, q);   // + am(row) * am(column)
+                q = fma(-r[u].x, c[u].x, q);  // - a(row) * a(column)
             }
         };
-        rd(0, cA, fA);
+        rd(0, rA, cA);
 #pragma unroll
         for (int t0 = 0; t0 < T; t0 += 2 * U) {
-            rd(t0 + U, cB, fB);
+            rd(t0 + U, rB, cB);
             __builtin_amdgcn_sched_barrier(0);
-            run(cA, fA);
+            run(rA, cA);
             __builtin_amdgcn_sched_barrier(0);
-            if (t0 + 2 * U < T) rd(t0 + 2 * U, cA, fA);
+            if (t0 + 2 * U < T) rd(t0 + 2 * U, rA, cA);
             __builtin_amdgcn_sched_barrier(0);
-            run(cB, fB);
+            run(rB, cB);
             __builtin_amdgcn_sched_barrier(0);
         }
         if ((j0 + T) % seg == 0 && d < blocklen) out[static_cast<int64_t>((j0 + T) / seg) * bw] = q;
         __builtin_amdgcn_wave_barrier();
         if (more) {
+            s_row[wv][buf ^ 1][lane] = n0;
+            s_row[wv][buf ^ 1][T + lane] = n1;
             s_col[wv][buf ^ 1][lane] = nc;
-            s_fill[wv][buf ^ 1][lane] = nf;
         }
         __builtin_amdgcn_wave_barrier();
     }
